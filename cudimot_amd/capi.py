@@ -1,0 +1,190 @@
+"""ctypes binding of the engine's C-ABI (``include/cudimot_amd.h``).
+
+Host plumbing only: device buffers are torch tensors (or raw device
+pointers); every call goes to ``lib/libcudimot.so``.  There is no CPU
+fallback: if the library is missing or a call fails, an exception is raised.
+"""
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libcudimot.so")
+
+GRIDSEARCH, LEVMAR, MCMC, PREDICTED, BIC_AIC = range(5)
+
+_lib = None
+
+
+class CdmError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CdmError("engine library not built: %s (run python -m cudimot_amd.build)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, u64, i64, cp = ctypes.c_void_p, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_longlong, ctypes.c_char_p
+        ip, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+        sig = {
+            "cdm_version": (i, []),
+            "cdm_last_error": (cp, []),
+            "cdm_list_models": (i, [cp, i]),
+            "cdm_model_open": (i, [cp, cp, ctypes.POINTER(vp)]),
+            "cdm_model_close": (None, [vp]),
+            "cdm_model_info": (i, [vp, ip, ip, ip]),
+            "cdm_model_spec": (i, [vp, ip, dp, dp, ip, dp, dp, ip, dp, ip]),
+            "cdm_model_set_fixed": (i, [vp, cp]),
+            "cdm_model_load_grid": (i, [vp, cp, i, ip, ip]),
+            "cdm_model_grid": (i, [vp, ip, dp]),
+            "cdm_model_set_group": (i, [vp, i, i]),
+            "cdm_model_get_group": (i, [vp, i, i]),
+            "cdm_model_set_mcmc_blocks": (i, [vp, i, i]),
+            "cdm_gridsearch": (i, [vp, i, i, vp, vp, vp, vp, vp, vp]),
+            "cdm_levmar": (i, [vp, i, i, vp, vp, vp, vp, i, i, vp, vp, vp]),
+            "cdm_mcmc": (i, [vp, i, i, vp, vp, vp, vp, i, i, i, i, i, u64, i64, vp, vp, vp]),
+            "cdm_predicted_signal": (i, [vp, i, i, i, vp, vp, vp, vp, vp]),
+            "cdm_bic_aic": (i, [vp, i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp]),
+            "cdm_rician_clean": (i, [i, i, vp]),
+            "cdm_sync": (i, [vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise CdmError(lib().cdm_last_error().decode())
+    return rc
+
+
+def ptr(x):
+    """Device pointer of a torch tensor (or an int pointer / None)."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def list_models():
+    n = lib().cdm_list_models(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    lib().cdm_list_models(buf, n + 1)
+    return [s for s in buf.value.decode().split(",") if s]
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+class Model:
+    """A compiled model plug-in plus its parsed modelpriors, grid and
+    --fixed list (the reference's Model<T>, Model.cu)."""
+
+    def __init__(self, name, priors_file=None):
+        self.name = name
+        h = ctypes.c_void_p()
+        check(lib().cdm_model_open(name.encode(), priors_file.encode() if priors_file else None, ctypes.byref(h)))
+        self.h = h
+        n, c, f = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().cdm_model_info(self.h, ctypes.byref(n), ctypes.byref(c), ctypes.byref(f)))
+        self.nparams, self.cfp_tsize, self.fixp_tsize = n.value, c.value, f.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cdm_model_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def spec(self):
+        n = self.nparams
+        I = ctypes.c_int * n
+        D = ctypes.c_double * n
+        bt, pt, fx = I(), I(), I()
+        lo, hi, pa, pb, init = D(), D(), D(), D(), D()
+        prov = ctypes.c_int()
+        check(lib().cdm_model_spec(self.h, bt, lo, hi, pt, pa, pb, fx, init, ctypes.byref(prov)))
+        return {"bound_types": list(bt), "bmin": list(lo), "bmax": list(hi), "prior_types": list(pt),
+                "pa": list(pa), "pb": list(pb), "fixed": list(fx), "p_init": list(init),
+                "init_provided": bool(prov.value)}
+
+    def set_fixed(self, ids):
+        s = ids if isinstance(ids, str) else ",".join(str(i) for i in ids)
+        check(lib().cdm_model_set_fixed(self.h, s.encode()))
+
+    def load_grid(self, src, is_text=False):
+        nc, ng = ctypes.c_int(), ctypes.c_int()
+        check(lib().cdm_model_load_grid(self.h, src.encode(), 1 if is_text else 0, ctypes.byref(nc),
+                                        ctypes.byref(ng)))
+        gp = (ctypes.c_int * max(ng.value, 1))()
+        g = (ctypes.c_double * max(nc.value * ng.value, 1))()
+        check(lib().cdm_model_grid(self.h, gp, g))
+        return list(gp)[:ng.value], [list(g)[k * ng.value:(k + 1) * ng.value] for k in range(nc.value)]
+
+    def set_group(self, method, lanes):
+        check(lib().cdm_model_set_group(self.h, method, lanes))
+
+    def get_group(self, method, nmeas):
+        r = lib().cdm_model_get_group(self.h, method, nmeas)
+        if r < 0:
+            raise CdmError(lib().cdm_last_error().decode())
+        return r
+
+    def set_mcmc_blocks(self, jumps_per_launch, use_graph=False):
+        check(lib().cdm_model_set_mcmc_blocks(self.h, int(jumps_per_launch), 1 if use_graph else 0))
+
+    # --- methods on device buffers (torch CUDA tensors, float32, contiguous)
+    def gridsearch(self, meas, params, cfp, fixp=None, kept=None, stream=None):
+        nvox, nmeas = meas.shape
+        check(lib().cdm_gridsearch(self.h, nvox, nmeas, ptr(meas), ptr(params), ptr(cfp), ptr(fixp), ptr(kept),
+                                   _stream(stream)))
+
+    def levmar(self, meas, params, cfp, fixp=None, max_iters=200, marquardt=True, iters=None, cost=None,
+               stream=None):
+        nvox, nmeas = meas.shape
+        check(lib().cdm_levmar(self.h, nvox, nmeas, ptr(meas), ptr(params), ptr(cfp), ptr(fixp), int(max_iters),
+                               1 if marquardt else 0, ptr(iters), ptr(cost), _stream(stream)))
+
+    def mcmc(self, meas, params, cfp, fixp, samples, tau_samples=None, nburn=1000, njumps=1250, sampleevery=25,
+             update_proposal=True, rician=False, seed=8219, voxel_offset=0, stream=None):
+        nvox, nmeas = meas.shape
+        check(lib().cdm_mcmc(self.h, nvox, nmeas, ptr(meas), ptr(params), ptr(cfp), ptr(fixp), int(nburn),
+                             int(njumps), int(sampleevery), 1 if update_proposal else 0, 1 if rician else 0,
+                             int(seed), int(voxel_offset), ptr(samples), ptr(tau_samples), _stream(stream)))
+
+    def predicted_signal(self, samples, cfp, fixp, out, stream=None):
+        nvox, _, ns = samples.shape
+        check(lib().cdm_predicted_signal(self.h, nvox, out.shape[1], ns, ptr(samples), ptr(cfp), ptr(fixp),
+                                         ptr(out), _stream(stream)))
+
+    def bic_aic(self, meas, samples, cfp, fixp, bic, aic, rician=False, tau_samples=None, stream=None):
+        nvox, nmeas = meas.shape
+        ns = samples.shape[2]
+        check(lib().cdm_bic_aic(self.h, nvox, nmeas, ns, 1 if rician else 0, ptr(meas), ptr(samples),
+                                ptr(tau_samples), ptr(cfp), ptr(fixp), ptr(bic), ptr(aic), _stream(stream)))
+
+
+def sync(stream=None):
+    check(lib().cdm_sync(_stream(stream)))
+
+
+def rician_clean(meas_np):
+    """In-place --rician cleanup of a host float32 array [nvox][nmeas]."""
+    import numpy as np
+    assert meas_np.dtype == np.float32 and meas_np.flags.c_contiguous
+    check(lib().cdm_rician_clean(meas_np.shape[0], meas_np.shape[1], meas_np.ctypes.data))

@@ -1,0 +1,95 @@
+// Plug-in math ABI for cudimot-style model plug-ins (mymodels/<name>/modelfunctions.h).
+//
+// A plug-in calls these names from its templated Predicted_Signal /
+// Partial_Derivatives / constraints code.  The names and their precision
+// contract follow the reference's device helper header (functions_gpu.h:6-40):
+//   * log/exp/fabs/sin/cos/atan/min/max keep the argument's precision;
+//   * tan/sqrt/atan2/acos/pow always return double, whatever the argument.
+// The second group is kept for source compatibility with existing plug-ins.
+// Plug-ins written for this engine use the T-preserving t* helpers below
+// (tsqrt, tatan2, tacos, tpow, ttan) so that an FP32 instantiation stays in
+// FP32 on the GPU.
+//
+// The same header compiles for the GPU (hipcc, device functions) and for the
+// host (g++, plain inline functions), so one plug-in source serves the HIP
+// kernels and the CPU code that checks them.
+#pragma once
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define CDM_FUNC __device__ __forceinline__
+#else
+#include <cmath>
+#include <math.h>
+#include <algorithm>
+#ifndef __device__
+#define __device__
+#endif
+#ifndef __host__
+#define __host__
+#endif
+#ifndef __forceinline__
+#define __forceinline__ inline
+#endif
+#define CDM_FUNC inline
+using std::max;
+using std::min;
+#endif
+
+CDM_FUNC float log_gpu(float x) { return logf(x); }
+CDM_FUNC double log_gpu(double x) { return log(x); }
+CDM_FUNC float exp_gpu(float x) { return expf(x); }
+CDM_FUNC double exp_gpu(double x) { return exp(x); }
+CDM_FUNC float fabs_gpu(float x) { return fabsf(x); }
+CDM_FUNC double fabs_gpu(double x) { return fabs(x); }
+CDM_FUNC float sin_gpu(float x) { return sinf(x); }
+CDM_FUNC double sin_gpu(double x) { return sin(x); }
+CDM_FUNC float cos_gpu(float x) { return cosf(x); }
+CDM_FUNC double cos_gpu(double x) { return cos(x); }
+CDM_FUNC float atan_gpu(float x) { return atanf(x); }
+CDM_FUNC double atan_gpu(double x) { return atan(x); }
+CDM_FUNC float min_gpu(float a, float b) { return fminf(a, b); }
+CDM_FUNC double min_gpu(double a, double b) { return fmin(a, b); }
+CDM_FUNC float max_gpu(float a, float b) { return fmaxf(a, b); }
+CDM_FUNC double max_gpu(double a, double b) { return fmax(a, b); }
+
+// Promoted to double for every argument type (reference contract).
+CDM_FUNC double tan_gpu(float x) { return tan((double)x); }
+CDM_FUNC double tan_gpu(double x) { return tan(x); }
+CDM_FUNC double sqrt_gpu(float x) { return sqrt((double)x); }
+CDM_FUNC double sqrt_gpu(double x) { return sqrt(x); }
+CDM_FUNC double atan2_gpu(float y, float x) { return atan2((double)y, (double)x); }
+CDM_FUNC double atan2_gpu(double y, double x) { return atan2(y, x); }
+CDM_FUNC double acos_gpu(float x) { return acos((double)x); }
+CDM_FUNC double acos_gpu(double x) { return acos(x); }
+CDM_FUNC double pow_gpu(float a, float b) { return pow((double)a, (double)b); }
+CDM_FUNC double pow_gpu(double a, double b) { return pow(a, b); }
+
+// T-preserving helpers (engine extension, no counterpart in the reference).
+CDM_FUNC float tsqrt(float x) { return sqrtf(x); }
+CDM_FUNC double tsqrt(double x) { return sqrt(x); }
+CDM_FUNC float tatan2(float y, float x) { return atan2f(y, x); }
+CDM_FUNC double tatan2(double y, double x) { return atan2(y, x); }
+CDM_FUNC float tacos(float x) { return acosf(x); }
+CDM_FUNC double tacos(double x) { return acos(x); }
+CDM_FUNC float ttan(float x) { return tanf(x); }
+CDM_FUNC double ttan(double x) { return tan(x); }
+CDM_FUNC float tlog(float x) { return logf(x); }
+CDM_FUNC double tlog(double x) { return log(x); }
+CDM_FUNC float texp(float x) { return expf(x); }
+CDM_FUNC double texp(double x) { return exp(x); }
+CDM_FUNC void tsincos(float x, float* s, float* c) { *s = sinf(x); *c = cosf(x); }
+CDM_FUNC void tsincos(double x, double* s, double* c) { *s = sin(x); *c = cos(x); }
+CDM_FUNC float terf(float x) { return erff(x); }
+CDM_FUNC double terf(double x) { return erf(x); }
+
+// Central-difference step for NUMERICAL(i) by precision (see
+// macro_numerical.h): the reference's constants for double, a wider relative
+// step and floor for float.
+template <typename T> struct cdm_fd_step;
+template <> struct cdm_fd_step<double> {
+  static constexpr double rel = 1e-5, lo = 1e-6, hi = 0.2;
+};
+template <> struct cdm_fd_step<float> {
+  static constexpr float rel = 2e-3f, lo = 1e-4f, hi = 0.2f;
+};

@@ -1,0 +1,874 @@
+// Fitting kernels, instantiated once per model plug-in.
+//
+// This header is included inside a model namespace right after the
+// plug-in's modelparameters.h and modelfunctions.h, so NPARAMS, MyType and
+// the plug-in functions resolve unqualified.  Identifiers here are long on
+// purpose: plug-in headers define short macros (H, MN, A1, ...).
+//
+// Work layout (all kernels): a block of 256 threads = 4 waves; a voxel group
+// of L lanes owns one voxel (L = 1..64, chosen per model and measurement
+// count); lane l of a group evaluates measurements l, l+L, ...  The per-
+// voxel state (parameters, transforms, lambda, chain state) lives in
+// registers, replicated in the L lanes of the group; sums over measurements
+// use cdm::Group<L>::sum.  Common fixed parameters (gradient directions,
+// b-values) are staged in LDS once per block.
+//
+// Precision: the plug-in's forward model and Jacobian run in MyType (FP32
+// for the shipped plug-ins); costs and log-likelihoods accumulate in FP64;
+// the LM normal equations are solved by an FP64 Cholesky factorisation;
+// chain and LM state are FP64.
+
+typedef MyType cdm_tf;
+constexpr int cdm_np = NPARAMS;
+constexpr int cdm_ntri = NPARAMS * (NPARAMS + 1) / 2;
+constexpr int cdm_cfp_max = 16;
+constexpr int cdm_fixp_max = 8;
+constexpr int cdm_block = 256;
+
+enum { cdm_NOBOUNDS = 0, cdm_BMIN = 1, cdm_BMAX = 2, cdm_BMINMAX = 3 };
+enum { cdm_NOPRIOR = 0, cdm_GAUSS = 1, cdm_GAMMA = 2, cdm_ARD = 3, cdm_SIN = 4, cdm_CUSTOM = 5 };
+
+// Bounds, priors and fixed flags, passed by value (the reference keeps them
+// in __constant__ memory as float: GridSearch.cu:20-23,
+// Levenberg_Marquardt.cu:28-31, MCMC.cu:24-30).
+struct cdm_consts {
+  int bound_types[cdm_np];
+  float bmin[cdm_np], bmax[cdm_np];
+  int prior_types[cdm_np];
+  float pa[cdm_np], pb[cdm_np];
+  int fixed[cdm_np];
+};
+
+struct cdm_geom {
+  int nvox, nmeas, cfp_ts, fixp_ts;
+  const float* meas;    // [nvox][nmeas]
+  const float* cfp;     // [nmeas][cfp_ts]
+  const float* fixp;    // [nvox][fixp_ts]
+};
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ void cdm_stage_cfp(const cdm_geom& g, float* s_cfp) {
+  const int n = g.nmeas * g.cfp_ts;
+  for (int i = threadIdx.x; i < n + cdm_cfp_max; i += blockDim.x) s_cfp[i] = i < n ? g.cfp[i] : 0.f;
+  __syncthreads();
+}
+
+template <class TF>
+__device__ __forceinline__ void cdm_load_fixp(const cdm_geom& g, int v, TF* fx) {
+#pragma unroll
+  for (int k = 0; k < cdm_fixp_max; ++k) fx[k] = k < g.fixp_ts ? (TF)g.fixp[(size_t)v * g.fixp_ts + k] : (TF)0;
+}
+
+// Pointer to one measurement's CFP row in the plug-in's precision.
+template <class TF>
+struct cdm_cfp_row {
+  TF buf[cdm_cfp_max];
+  __device__ __forceinline__ TF* at(const float* s_cfp, int m, int cts) {
+#pragma unroll
+    for (int k = 0; k < cdm_cfp_max; ++k) buf[k] = (TF)s_cfp[m * cts + k];
+    return buf;
+  }
+};
+template <>
+struct cdm_cfp_row<float> {
+  __device__ __forceinline__ float* at(const float* s_cfp, int m, int cts) {
+    return const_cast<float*>(&s_cfp[m * cts]);
+  }
+};
+
+// Sum of squared residuals for parameters p (converted to TF), FP64 sums.
+template <int L, class TF>
+__device__ __forceinline__ double cdm_sse(const cdm_geom& g, const float* s_cfp, const float* mv, TF* p, TF* fx,
+                                          int lane) {
+  double acc = 0.0;
+  cdm_cfp_row<TF> row;
+  for (int m = lane; m < g.nmeas; m += L) {
+    const TF s = Predicted_Signal(cdm_np, p, row.at(s_cfp, m, g.cfp_ts), fx);
+    const TF r = s - (TF)mv[m];
+    acc += (double)(r * r);
+  }
+  return cdm::Group<L>::sum(acc);
+}
+
+// ---------------------------------------------------------------- GridSearch
+// GridSearch.cu:97-248.  Combinations whose grid values fall outside the
+// float bounds are removed on the host (same test as checkBounds,
+// GridSearch.cu:25-47); the kernel tests the non-grid parameters once per
+// voxel.  An FP32 screen keeps the CDM_GS_TOPK best combinations in
+// enumeration order (strict <, so ties keep the earliest); those are
+// re-scored in FP64 and the FP64 minimum (earliest on ties) is kept.  When
+// more than CDM_GS_TOPK combinations sit within the FP32 error band of the
+// best, the voxel rescans every combination in FP64, so the chosen index is
+// the FP64 argmin.
+
+#ifndef CDM_GS_TOPK
+#define CDM_GS_TOPK 4
+#endif
+
+struct cdm_gs_args {
+  cdm_geom g;
+  float* params;            // [nvox][NP] in/out
+  int ngp;                  // grid parameters
+  int gparams[cdm_np];      // their indices, ascending
+  int ncombs;               // valid (in-bounds) combinations
+  const float* grid_f;      // [ncombs][ngp]
+  const double* grid_d;     // [ncombs][ngp]
+  const int* comb_id;       // [ncombs] original enumeration index
+  cdm_consts c;
+  int* kept_out;            // [nvox] kept original index or -1 (nullable)
+};
+
+template <int L>
+__global__ void __launch_bounds__(cdm_block) cdm_gridsearch_kernel(cdm_gs_args a) {
+  extern __shared__ float s_cfp[];
+  cdm_stage_cfp(a.g, s_cfp);
+  const int v = blockIdx.x * (cdm_block / L) + threadIdx.x / L;
+  const int lane = threadIdx.x % L;
+  if (v >= a.g.nvox) return;
+  const float* mv = a.g.meas + (size_t)v * a.g.nmeas;
+
+  float base[cdm_np];
+  bool is_grid[cdm_np];
+#pragma unroll
+  for (int i = 0; i < cdm_np; ++i) {
+    base[i] = a.params[(size_t)v * cdm_np + i];
+    is_grid[i] = false;
+  }
+  for (int k = 0; k < a.ngp; ++k) {
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i)
+      if (a.gparams[k] == i) is_grid[i] = true;
+  }
+  bool ok = a.ncombs > 0;
+#pragma unroll
+  for (int i = 0; i < cdm_np; ++i) {
+    if (is_grid[i]) continue;
+    const double x = base[i];
+    const int bt = a.c.bound_types[i];
+    if ((bt == cdm_BMIN || bt == cdm_BMINMAX) && x < a.c.bmin[i]) ok = false;
+    if ((bt == cdm_BMAX || bt == cdm_BMINMAX) && x > a.c.bmax[i]) ok = false;
+  }
+  if (!ok) {
+    if (lane == 0 && a.kept_out) a.kept_out[v] = -1;
+    return;
+  }
+  cdm_tf fx[cdm_fixp_max];
+  cdm_load_fixp(a.g, v, fx);
+
+  // FP32 (MyType) screen with a sorted top-K list.
+  float best_c[CDM_GS_TOPK];
+  int best_i[CDM_GS_TOPK];
+#pragma unroll
+  for (int k = 0; k < CDM_GS_TOPK; ++k) {
+    best_c[k] = INFINITY;
+    best_i[k] = -1;
+  }
+  double sumsq = 0.0;
+  for (int m = lane; m < a.g.nmeas; m += L) sumsq += (double)mv[m] * (double)mv[m];
+  sumsq = cdm::Group<L>::sum(sumsq);
+
+  cdm_tf trial[cdm_np];
+  for (int c = 0; c < a.ncombs; ++c) {
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) trial[i] = (cdm_tf)base[i];
+    for (int k = 0; k < a.ngp; ++k) {
+      const float gv = a.grid_f[(size_t)c * a.ngp + k];
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i)
+        if (i == a.gparams[k]) trial[i] = (cdm_tf)gv;
+    }
+    const float cost = (float)cdm_sse<L>(a.g, s_cfp, mv, trial, fx, lane);
+    if (cost < best_c[CDM_GS_TOPK - 1]) {
+      // insertion keeps (cost, index) order; equal costs keep the earlier index first
+      float cc = cost;
+      int ci = c;
+#pragma unroll
+      for (int k = 0; k < CDM_GS_TOPK; ++k) {
+        if (cc < best_c[k]) {
+          const float tc = best_c[k];
+          const int ti = best_i[k];
+          best_c[k] = cc;
+          best_i[k] = ci;
+          cc = tc;
+          ci = ti;
+        }
+      }
+    }
+  }
+
+  int chosen = best_i[0];
+  if constexpr (sizeof(cdm_tf) != sizeof(double)) {
+    // Error band of the FP32 cost (rounding of the forward model and of the
+    // residual sum), generous by a factor ~10.
+    const double b0 = best_c[0];
+    const double band = 4e-5 * sqrt(fmax(b0, 0.0) * sumsq) + 1e-4 * b0 + 1e-8 * sumsq;
+    const bool crowded = best_i[CDM_GS_TOPK - 1] >= 0 && (double)best_c[CDM_GS_TOPK - 1] <= b0 + band;
+    // Candidates in ascending enumeration order (empty slots last).
+    int cand[CDM_GS_TOPK];
+#pragma unroll
+    for (int k = 0; k < CDM_GS_TOPK; ++k) cand[k] = best_i[k] < 0 ? INT_MAX : best_i[k];
+#pragma unroll
+    for (int i = 0; i < CDM_GS_TOPK; ++i)
+#pragma unroll
+      for (int k = 0; k + 1 < CDM_GS_TOPK - i; ++k)
+        if (cand[k] > cand[k + 1]) {
+          const int tmp = cand[k];
+          cand[k] = cand[k + 1];
+          cand[k + 1] = tmp;
+        }
+    double trial_d[cdm_np];
+    double fx_d[cdm_fixp_max];
+#pragma unroll
+    for (int k = 0; k < cdm_fixp_max; ++k) fx_d[k] = (double)fx[k];
+    double bestd = INFINITY;
+    int bestd_i = -1;
+    const int n_rescore = crowded ? a.ncombs : CDM_GS_TOPK;
+    for (int q = 0; q < n_rescore; ++q) {
+      int c = q;
+      if (!crowded) {
+#pragma unroll
+        for (int k = 0; k < CDM_GS_TOPK; ++k)
+          if (k == q) c = cand[k];
+        if (c == INT_MAX) break;
+      }
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i) trial_d[i] = (double)base[i];
+      for (int k = 0; k < a.ngp; ++k) {
+        const double gv = a.grid_d[(size_t)c * a.ngp + k];
+#pragma unroll
+        for (int i = 0; i < cdm_np; ++i)
+          if (i == a.gparams[k]) trial_d[i] = gv;
+      }
+      const double cd = cdm_sse<L>(a.g, s_cfp, mv, trial_d, fx_d, lane);
+      if (cd < bestd) {
+        bestd = cd;
+        bestd_i = c;
+      }
+    }
+    chosen = bestd_i;
+  }
+  if (chosen < 0) {
+    if (lane == 0 && a.kept_out) a.kept_out[v] = -1;
+    return;
+  }
+  if (lane == 0) {
+    for (int k = 0; k < a.ngp; ++k) a.params[(size_t)v * cdm_np + a.gparams[k]] = a.grid_f[(size_t)chosen * a.ngp + k];
+    if (a.kept_out) a.kept_out[v] = a.comb_id[chosen];
+  }
+}
+
+// ------------------------------------------------------ Levenberg-Marquardt
+// Levenberg_Marquardt.cu:409-623, with the bound transforms of :37-135 in
+// FP64 against float bounds.  The gradient and Hessian come from one fused
+// pass (the reference makes two passes over the same values); each lane
+// accumulates its measurements in FP32, group sums are FP32, and the damped
+// normal equations are solved by FP64 Cholesky.  A failed factorisation
+// (non-positive pivot) yields a NaN step, which the cost test rejects, as
+// the reference's zero pivot does.  Only successful iterations count toward
+// max_iters; the loop also ends on a relative cost change <= 1e-8 or
+// lambda > 1e20.
+
+constexpr double cdm_lm_cftol = 1.0e-8, cdm_lm_ltol = 1.0e20, cdm_lm_eps = 2.0e-16;
+constexpr double cdm_lm_twodivpi = 0.636619772367581, cdm_lm_avoid = 1e-4, cdm_lm_speed = 10;
+
+struct cdm_lm_args {
+  cdm_geom g;
+  float* params;  // [nvox][NP] in/out
+  int max_iters;
+  cdm_consts c;
+  int* iters_out;       // successful iterations per voxel (nullable)
+  float* cost_out;      // final cost per voxel (nullable)
+};
+
+__device__ __forceinline__ void cdm_minmax_t(double lo, double hi, double p, double& t) {
+  t = p - (lo + hi) / 2.0;
+  t *= 2.0 / ((hi - lo) * cdm_lm_twodivpi);
+  t = tan(t) * cdm_lm_speed;
+}
+
+__device__ __forceinline__ void cdm_inv_transform(const cdm_consts& c, double* p, double* t) {
+#pragma unroll
+  for (int i = 0; i < cdm_np; ++i) {
+    const int bt = c.bound_types[i];
+    const double lo = c.bmin[i], hi = c.bmax[i];
+    if (bt == cdm_BMIN) {
+      t[i] = log(p[i] - lo);
+    } else if (bt == cdm_BMAX) {
+      t[i] = log(hi - p[i]);
+    } else if (bt == cdm_BMINMAX) {
+      if (p[i] <= lo + cdm_lm_avoid) p[i] += cdm_lm_avoid;
+      if (p[i] >= hi - cdm_lm_avoid) p[i] -= cdm_lm_avoid;
+      cdm_minmax_t(lo, hi, p[i], t[i]);
+    } else {
+      t[i] = p[i];
+    }
+  }
+}
+
+__device__ __forceinline__ void cdm_transform(const cdm_consts& c, double* p, double* t) {
+#pragma unroll
+  for (int i = 0; i < cdm_np; ++i) {
+    const int bt = c.bound_types[i];
+    const double lo = c.bmin[i], hi = c.bmax[i];
+    if (bt == cdm_BMIN) {
+      p[i] = exp(t[i]) + lo;
+    } else if (bt == cdm_BMAX) {
+      p[i] = hi - exp(t[i]);
+    } else if (bt == cdm_BMINMAX) {
+      double x = (hi - lo) / 2.0;
+      x *= atan(t[i] / cdm_lm_speed);
+      x *= cdm_lm_twodivpi;
+      x += (lo + hi) / 2.0;
+      if (x <= lo + cdm_lm_avoid) {
+        x += cdm_lm_avoid;
+        cdm_minmax_t(lo, hi, x, t[i]);
+      }
+      if (x >= hi - cdm_lm_avoid) {
+        x -= cdm_lm_avoid;
+        cdm_minmax_t(lo, hi, x, t[i]);
+      }
+      p[i] = x;
+    } else {
+      p[i] = t[i];
+    }
+  }
+}
+
+// d(param)/d(transformed param), Levenberg_Marquardt.cu:120-135.
+__device__ __forceinline__ void cdm_dtransform(const cdm_consts& c, const double* t, double* d) {
+#pragma unroll
+  for (int i = 0; i < cdm_np; ++i) {
+    const int bt = c.bound_types[i];
+    if (bt == cdm_BMIN) {
+      d[i] = exp(t[i]);
+    } else if (bt == cdm_BMAX) {
+      d[i] = -exp(t[i]);
+    } else if (bt == cdm_BMINMAX) {
+      const float span = c.bmax[i] - c.bmin[i];  // float difference, as the reference
+      d[i] = (span / 2.0) * cdm_lm_twodivpi * (cdm_lm_speed / (cdm_lm_speed * cdm_lm_speed + t[i] * t[i]));
+    } else {
+      d[i] = 1.0;
+    }
+  }
+}
+
+template <int L>
+__global__ void __launch_bounds__(cdm_block) cdm_levmar_kernel(cdm_lm_args a, int marquardt) {
+  extern __shared__ float s_cfp[];
+  cdm_stage_cfp(a.g, s_cfp);
+  const int v = blockIdx.x * (cdm_block / L) + threadIdx.x / L;
+  const int lane = threadIdx.x % L;
+  if (v >= a.g.nvox) return;
+  const float* mv = a.g.meas + (size_t)v * a.g.nmeas;
+  cdm_tf fx[cdm_fixp_max];
+  cdm_load_fixp(a.g, v, fx);
+
+  double p[cdm_np], t[cdm_np];
+#pragma unroll
+  for (int i = 0; i < cdm_np; ++i) p[i] = a.params[(size_t)v * cdm_np + i];
+  cdm_inv_transform(a.c, p, t);
+
+  cdm_tf pf[cdm_np];
+#pragma unroll
+  for (int i = 0; i < cdm_np; ++i) pf[i] = (cdm_tf)p[i];
+  double pcf = cdm_sse<L>(a.g, s_cfp, mv, pf, fx, lane);
+
+  double lambda = 0.1, olambda = 0.0;
+  bool success = true;
+  int iter = 0, nsucc = 0;
+  float hs[cdm_ntri];   // 2 J^T J, packed lower triangle (undamped)
+  double grad[cdm_np];  // 2 J^T r
+  double diag[cdm_np];  // current damped diagonal
+  cdm_cfp_row<cdm_tf> row;
+
+  while (!(success && iter++ >= a.max_iters)) {
+    if (success) {
+      double dt[cdm_np];
+      cdm_dtransform(a.c, t, dt);
+      cdm_tf dtf[cdm_np];
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i) dtf[i] = (cdm_tf)dt[i];
+      float gl[cdm_np];
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i) gl[i] = 0.f;
+#pragma unroll
+      for (int k = 0; k < cdm_ntri; ++k) hs[k] = 0.f;
+      for (int m = lane; m < a.g.nmeas; m += L) {
+        cdm_tf* cm = row.at(s_cfp, m, a.g.cfp_ts);
+        const cdm_tf r = Predicted_Signal(cdm_np, pf, cm, fx) - (cdm_tf)mv[m];
+        cdm_tf d[cdm_np];
+#pragma unroll
+        for (int i = 0; i < cdm_np; ++i) d[i] = (cdm_tf)0;
+        Partial_Derivatives(cdm_np, pf, cm, fx, d);
+        float df[cdm_np];
+#pragma unroll
+        for (int i = 0; i < cdm_np; ++i) df[i] = (float)(d[i] * dtf[i]);
+        const float rf = (float)r;
+#pragma unroll
+        for (int i = 0; i < cdm_np; ++i) gl[i] += rf * df[i];
+        int k = 0;
+#pragma unroll
+        for (int i = 0; i < cdm_np; ++i)
+#pragma unroll
+          for (int j = 0; j <= i; ++j) hs[k++] += df[i] * df[j];
+      }
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i) {
+        gl[i] = cdm::Group<L>::sum(gl[i]);
+        grad[i] = a.c.fixed[i] ? 0.0 : 2.0 * (double)gl[i];
+      }
+      int k = 0;
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j, ++k) {
+          float h = 2.f * cdm::Group<L>::sum(hs[k]);
+          if (a.c.fixed[i] || a.c.fixed[j]) h = (i == j) ? 1.f : 0.f;
+          hs[k] = h;
+        }
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i) diag[i] = hs[i * (i + 1) / 2 + i];
+    }
+    // Damping applied cumulatively to the kept diagonal, as the reference
+    // damps its shared Hessian in place (Levenberg_Marquardt.cu:530-537).
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) {
+      if (marquardt) diag[i] = ((1 + lambda) / (1 + olambda)) * diag[i];
+      else diag[i] += lambda - olambda;
+    }
+    // FP64 Cholesky of the damped matrix, then two triangular solves.
+    double fac[cdm_ntri];
+    bool spd = true;
+    {
+      int k = 0;
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j, ++k) fac[k] = (i == j) ? diag[i] : (double)hs[k];
+#pragma unroll
+      for (int j = 0; j < cdm_np; ++j) {
+        double s = fac[j * (j + 1) / 2 + j];
+#pragma unroll
+        for (int q = 0; q < j; ++q) s -= fac[j * (j + 1) / 2 + q] * fac[j * (j + 1) / 2 + q];
+        spd = spd && (s > 0.0);
+        const double ljj = sqrt(s);
+        fac[j * (j + 1) / 2 + j] = ljj;
+        const double inv = 1.0 / ljj;
+#pragma unroll
+        for (int i = j + 1; i < cdm_np; ++i) {
+          double x = fac[i * (i + 1) / 2 + j];
+#pragma unroll
+          for (int q = 0; q < j; ++q) x -= fac[i * (i + 1) / 2 + q] * fac[j * (j + 1) / 2 + q];
+          fac[i * (i + 1) / 2 + j] = x * inv;
+        }
+      }
+    }
+    double step[cdm_np];
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) {
+      double x = grad[i];
+#pragma unroll
+      for (int q = 0; q < i; ++q) x -= fac[i * (i + 1) / 2 + q] * step[q];
+      step[i] = x / fac[i * (i + 1) / 2 + i];
+    }
+#pragma unroll
+    for (int i = cdm_np - 1; i >= 0; --i) {
+      double x = step[i];
+#pragma unroll
+      for (int q = i + 1; q < cdm_np; ++q) x -= fac[q * (q + 1) / 2 + i] * step[q];
+      step[i] = x / fac[i * (i + 1) / 2 + i];
+    }
+    double tn[cdm_np];
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) tn[i] = spd ? t[i] - step[i] : NAN;
+    cdm_transform(a.c, p, tn);
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) pf[i] = (cdm_tf)p[i];
+    const double ncf = cdm_sse<L>(a.g, s_cfp, mv, pf, fx, lane);
+    bool end = false;
+    success = ncf < pcf;
+    if (success) {
+      olambda = 0.0;
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i) t[i] = tn[i];
+      lambda = lambda / 10.0;
+      if (2.0 * fabs(pcf - ncf) <= cdm_lm_cftol * (fabs(pcf) + fabs(ncf) + cdm_lm_eps)) end = true;
+      pcf = ncf;
+      ++nsucc;
+    } else {
+      olambda = lambda;
+      lambda = lambda * 10.0;
+      if (lambda > cdm_lm_ltol) end = true;
+      cdm_transform(a.c, p, t);
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i) pf[i] = (cdm_tf)p[i];
+    }
+    if (end) break;
+  }
+  FixConstraintsLM(cdm_np, p);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) a.params[(size_t)v * cdm_np + i] = (float)p[i];
+    if (a.iters_out) a.iters_out[v] = nsucc;
+    if (a.cost_out) a.cost_out[v] = (float)pcf;
+  }
+}
+
+// --------------------------------------------------------------------- MCMC
+// MCMC.cu:327-683: single-site Metropolis-within-Gibbs with adaptive
+// proposal widths.  One launch runs `count` jumps of one phase (burn-in or
+// recording) starting at jump `begin` of that phase; the chain state is kept
+// in FP64 in `state` between launches, so a phase may be split into jump
+// blocks (captured in a HIP graph by the host) without changing any draw.
+// Phase initialisation (begin == 0) follows the reference's kernel prologue
+// (propSD = param/10 in burn-in, clamp to bounds, priors, tau); the phase's
+// last block writes params back.  Draws: Philox counters (philox.h) with
+// jump = phase offset + iteration.
+
+struct cdm_chain {  // FP64 chain state per voxel
+  double p[cdm_np], prior[cdm_np], sd[cdm_np];
+  double energy, tau, tau_sd;
+  int nacc[cdm_np], nrej[cdm_np];
+  int tau_acc, tau_rej;
+};
+
+struct cdm_mcmc_args {
+  cdm_geom g;
+  float* params;         // [nvox][NP] in/out
+  cdm_chain* state;      // [nvox]
+  float* samples;        // [nvox][NP][nsamples] (recording)
+  float* tau_samples;    // [nvox][nsamples] (Rician, recording)
+  int nsamples, sampleevery, upe;
+  int recording, update_prop;
+  int begin, count, phase_len;
+  unsigned jump_offset;  // Philox jump index of iteration 0 of this phase
+  unsigned long long seed;
+  long long voxel_offset;
+  cdm_consts c;
+};
+
+__device__ __forceinline__ double cdm_logI0(double x) {
+  // Numerical Recipes polynomial fits, as MCMC.cu:52-68.
+  double y, b = fabs(x);
+  if (b < 3.75) {
+    double q = x / 3.75;
+    q *= q;
+    y = 1.0 + q * (3.5156229 + q * (3.0899424 + q * (1.2067492 + q * (0.2659732 + q * (0.0360768 + q * 0.0045813)))));
+    y = log(y);
+  } else {
+    const double q = 3.75 / b;
+    y = b + log((0.39894228 +
+                 q * (0.01328592 +
+                      q * (0.00225319 +
+                           q * (-0.00157565 +
+                                q * (0.00916281 + q * (-0.02057706 + q * (0.02635537 + q * (-0.01647633 + q * 0.00392377)))))))) /
+                sqrt(b));
+  }
+  return y;
+}
+
+template <int L, bool RICIAN>
+__device__ __forceinline__ double cdm_neg_loglik(const cdm_geom& g, const float* s_cfp, const float* mv,
+                                                 const double* p, double tau, cdm_tf* fx, int lane) {
+  cdm_tf pf[cdm_np];
+#pragma unroll
+  for (int i = 0; i < cdm_np; ++i) pf[i] = (cdm_tf)p[i];
+  cdm_cfp_row<cdm_tf> row;
+  double acc = 0.0;
+  for (int m = lane; m < g.nmeas; m += L) {
+    const cdm_tf s = Predicted_Signal(cdm_np, pf, row.at(s_cfp, m, g.cfp_ts), fx);
+    if constexpr (RICIAN) {
+      const double y = mv[m], sd = (double)s;
+      acc += log(y) + (-0.5 * tau * (y * y + sd * sd) + cdm_logI0(tau * sd * y));
+    } else {
+      const cdm_tf r = s - (cdm_tf)mv[m];
+      acc += (double)(r * r);
+    }
+  }
+  acc = cdm::Group<L>::sum(acc);
+  if constexpr (RICIAN) return -g.nmeas * log(tau) - acc;
+  else return (g.nmeas / 2.0) * log(acc / 2.0);
+}
+
+template <class TP>
+__device__ __forceinline__ double cdm_prior(const cdm_consts& c, int i, TP* p, int nmeas, float* cfp, cdm_tf* fx) {
+  const int pt = c.prior_types[i];
+  if (pt == cdm_GAUSS) {
+    const double sd2 = c.pb[i] * c.pb[i];
+    return (p[i] - c.pa[i]) * (p[i] - c.pa[i]) / (2 * sd2);
+  } else if (pt == cdm_GAMMA) {
+    return (1.0 - c.pa[i]) * log(p[i]) + c.pb[i] * p[i];
+  } else if (pt == cdm_ARD) {
+    return c.pa[i] * log(p[i]);
+  } else if (pt == cdm_SIN) {
+    return -log(fabs(sin(p[i]) / 2.0));
+  } else if (pt == cdm_CUSTOM) {
+    cdm_tf pf[cdm_np];
+#pragma unroll
+    for (int k = 0; k < cdm_np; ++k) pf[k] = (cdm_tf)p[k];
+    cdm_cfp_row<cdm_tf> row;
+    return (double)custom_priors(i, pf, nmeas, row.at(cfp, 0, 0), fx);
+  }
+  return 0.0;
+}
+
+template <int L, bool RICIAN>
+__global__ void __launch_bounds__(cdm_block) cdm_mcmc_kernel(cdm_mcmc_args a) {
+  extern __shared__ float s_cfp[];
+  cdm_stage_cfp(a.g, s_cfp);
+  const int v = blockIdx.x * (cdm_block / L) + threadIdx.x / L;
+  const int lane = threadIdx.x % L;
+  if (v >= a.g.nvox) return;
+  const float* mv = a.g.meas + (size_t)v * a.g.nmeas;
+  cdm_tf fx[cdm_fixp_max];
+  cdm_load_fixp(a.g, v, fx);
+  const uint64_t gvox = (uint64_t)(a.voxel_offset + v);
+  cdm_chain* st = &a.state[v];
+
+  double p[cdm_np], prior[cdm_np], sd[cdm_np];
+  int nacc[cdm_np], nrej[cdm_np];
+  double energy, tau = 0.0, tau_sd = 0.0;
+  int tau_acc = 0, tau_rej = 0;
+
+  if (a.begin == 0) {
+    // Phase prologue (MCMC.cu:411-494).
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) {
+      p[i] = a.recording ? st->p[i] : (double)a.params[(size_t)v * cdm_np + i];
+      nacc[i] = nrej[i] = 0;
+      sd[i] = a.recording ? st->sd[i] : p[i] / 10.0;
+    }
+    if (RICIAN && a.recording) {
+      tau = st->tau;
+      tau_sd = st->tau_sd;
+    }
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) {
+      const int bt = a.c.bound_types[i];
+      if (bt == cdm_BMIN) {
+        if (p[i] < a.c.bmin[i]) p[i] = a.c.bmin[i];
+      } else if (bt == cdm_BMAX) {
+        if (p[i] > a.c.bmax[i]) p[i] = a.c.bmax[i];
+      } else if (bt == cdm_BMINMAX) {
+        if (p[i] < a.c.bmin[i]) p[i] = a.c.bmin[i];
+        else if (p[i] > a.c.bmax[i]) p[i] = a.c.bmax[i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) prior[i] = cdm_prior(a.c, i, p, a.g.nmeas, s_cfp, fx);
+    if (RICIAN && !a.recording) {
+      // tau = 1 / variance of the residuals (MCMC.cu:193-251)
+      cdm_tf pf[cdm_np];
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i) pf[i] = (cdm_tf)p[i];
+      cdm_cfp_row<cdm_tf> row;
+      double s1 = 0.0;
+      for (int m = lane; m < a.g.nmeas; m += L)
+        s1 += (double)(Predicted_Signal(cdm_np, pf, row.at(s_cfp, m, a.g.cfp_ts), fx) - (cdm_tf)mv[m]);
+      const double mean = cdm::Group<L>::sum(s1) / a.g.nmeas;
+      double s2 = 0.0;
+      for (int m = lane; m < a.g.nmeas; m += L) {
+        const double e = mean - (double)(Predicted_Signal(cdm_np, pf, row.at(s_cfp, m, a.g.cfp_ts), fx) -
+                                         (cdm_tf)mv[m]);
+        s2 += e * e;
+      }
+      tau = 1.0 / (cdm::Group<L>::sum(s2) / a.g.nmeas);
+      tau_sd = tau / 2.0;
+    }
+    double tp = 0.0;
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) tp += prior[i];
+    energy = tp + cdm_neg_loglik<L, RICIAN>(a.g, s_cfp, mv, p, tau, fx, lane);
+  } else {
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) {
+      p[i] = st->p[i];
+      prior[i] = st->prior[i];
+      sd[i] = st->sd[i];
+      nacc[i] = st->nacc[i];
+      nrej[i] = st->nrej[i];
+    }
+    energy = st->energy;
+    tau = st->tau;
+    tau_sd = st->tau_sd;
+    tau_acc = st->tau_acc;
+    tau_rej = st->tau_rej;
+  }
+
+  const int end = a.begin + a.count;
+  for (int it = a.begin; it < end; ++it) {
+    const uint32_t jump = a.jump_offset + (uint32_t)it;
+    if constexpr (RICIAN) {
+      const cdm::McmcDraw d = cdm::mcmc_draw(a.seed, gvox, jump, (uint32_t)cdm_np);
+      const double old = tau;
+      tau = tau + d.normal * tau_sd;
+      if (tau > 0.0) {
+        double tp = 0.0;
+#pragma unroll
+        for (int i = 0; i < cdm_np; ++i) tp += prior[i];
+        const double e_new = tp + cdm_neg_loglik<L, true>(a.g, s_cfp, mv, p, tau, fx, lane);
+        if (exp(energy - e_new) > d.uniform) {
+          ++tau_acc;
+          energy = e_new;
+        } else {
+          ++tau_rej;
+          tau = old;
+        }
+      } else {
+        ++tau_rej;
+        tau = old;
+      }
+    }
+#pragma unroll 1
+    for (int i = 0; i < cdm_np; ++i) {
+      const cdm::McmcDraw d = cdm::mcmc_draw(a.seed, gvox, jump, (uint32_t)i);
+      const double old = p[i];
+      if (!a.c.fixed[i]) p[i] = p[i] + d.normal * sd[i];
+      bool ok = ConstraintsMCMC(cdm_np, p);
+      const int bt = a.c.bound_types[i];
+      if ((bt == cdm_BMIN || bt == cdm_BMINMAX) && p[i] < a.c.bmin[i]) ok = false;
+      if ((bt == cdm_BMAX || bt == cdm_BMINMAX) && p[i] > a.c.bmax[i]) ok = false;
+      if (ok) {
+        const double like = cdm_neg_loglik<L, RICIAN>(a.g, s_cfp, mv, p, tau, fx, lane);
+        const double old_prior = prior[i];
+        prior[i] = cdm_prior(a.c, i, p, a.g.nmeas, s_cfp, fx);
+        double tp = 0.0;
+#pragma unroll
+        for (int k = 0; k < cdm_np; ++k) tp += prior[k];
+        const double e_new = tp + like;
+        if (exp(energy - e_new) > d.uniform) {
+          ++nacc[i];
+          energy = e_new;
+        } else {
+          ++nrej[i];
+          p[i] = old;
+          prior[i] = old_prior;
+        }
+      } else {
+        ++nrej[i];
+        p[i] = old;
+      }
+    }
+    if (a.recording && (it % a.sampleevery) == 0 && lane == 0) {
+      const int k = it / a.sampleevery;
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i) a.samples[((size_t)v * cdm_np + i) * a.nsamples + k] = (float)p[i];
+      if (RICIAN) a.tau_samples[(size_t)v * a.nsamples + k] = (float)tau;
+    }
+    if ((!a.recording || a.update_prop) && it > 0 && (it % a.upe) == 0) {
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i) {
+        sd[i] *= sqrt((nacc[i] + 1.0) / (nrej[i] + 1.0));
+        sd[i] = fmin(sd[i], 1e10);
+        nacc[i] = nrej[i] = 0;
+      }
+      if (RICIAN) {
+        tau_sd *= sqrt((tau_acc + 1.0) / (tau_rej + 1.0));
+        tau_sd = fmin(tau_sd, 1e10);
+        tau_acc = tau_rej = 0;
+      }
+    }
+  }
+
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) {
+      st->p[i] = p[i];
+      st->prior[i] = prior[i];
+      st->sd[i] = sd[i];
+      st->nacc[i] = nacc[i];
+      st->nrej[i] = nrej[i];
+    }
+    st->energy = energy;
+    st->tau = tau;
+    st->tau_sd = tau_sd;
+    st->tau_acc = tau_acc;
+    st->tau_rej = tau_rej;
+    if (end == a.phase_len) {
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i) a.params[(size_t)v * cdm_np + i] = (float)p[i];
+      if (RICIAN && !a.recording && a.tau_samples) a.tau_samples[(size_t)v * a.nsamples] = (float)tau;
+    }
+  }
+}
+
+// ------------------------------------------------- predicted signal, BIC/AIC
+// getPredictedSignal.cu:22-92 and BIC_AIC.cu:61-201: the arithmetic mean of
+// the samples (angles included) drives one forward evaluation per
+// measurement.
+
+struct cdm_post_args {
+  cdm_geom g;
+  int nsamples;
+  const float* samples;      // [nvox][NP][nsamples]
+  const float* tau_samples;  // [nvox][nsamples] (Rician BIC)
+  float* out;                // predicted [nvox][nmeas]
+  float* bic;                // [nvox]
+  float* aic;                // [nvox]
+};
+
+__device__ __forceinline__ void cdm_mean_sample(const cdm_post_args& a, int v, double* mp) {
+#pragma unroll
+  for (int i = 0; i < cdm_np; ++i) {
+    const float* s = a.samples + ((size_t)v * cdm_np + i) * a.nsamples;
+    if (a.nsamples > 1) {
+      double acc = 0.0;
+      for (int k = 0; k < a.nsamples; ++k) acc += s[k];
+      mp[i] = acc / a.nsamples;
+    } else {
+      mp[i] = s[0];
+    }
+  }
+}
+
+template <int L>
+__global__ void __launch_bounds__(cdm_block) cdm_predicted_kernel(cdm_post_args a) {
+  extern __shared__ float s_cfp[];
+  cdm_stage_cfp(a.g, s_cfp);
+  const int v = blockIdx.x * (cdm_block / L) + threadIdx.x / L;
+  const int lane = threadIdx.x % L;
+  if (v >= a.g.nvox) return;
+  cdm_tf fx[cdm_fixp_max];
+  cdm_load_fixp(a.g, v, fx);
+  double mp[cdm_np];
+  cdm_mean_sample(a, v, mp);
+  cdm_tf pf[cdm_np];
+#pragma unroll
+  for (int i = 0; i < cdm_np; ++i) pf[i] = (cdm_tf)mp[i];
+  cdm_cfp_row<cdm_tf> row;
+  for (int m = lane; m < a.g.nmeas; m += L)
+    a.out[(size_t)v * a.g.nmeas + m] = (float)Predicted_Signal(cdm_np, pf, row.at(s_cfp, m, a.g.cfp_ts), fx);
+}
+
+template <int L, bool RICIAN>
+__global__ void __launch_bounds__(cdm_block) cdm_bic_aic_kernel(cdm_post_args a) {
+  extern __shared__ float s_cfp[];
+  cdm_stage_cfp(a.g, s_cfp);
+  const int v = blockIdx.x * (cdm_block / L) + threadIdx.x / L;
+  const int lane = threadIdx.x % L;
+  if (v >= a.g.nvox) return;
+  const float* mv = a.g.meas + (size_t)v * a.g.nmeas;
+  cdm_tf fx[cdm_fixp_max];
+  cdm_load_fixp(a.g, v, fx);
+  double mp[cdm_np];
+  cdm_mean_sample(a, v, mp);
+  double tau = 0.0;
+  if (RICIAN) {
+    const float* ts = a.tau_samples + (size_t)v * a.nsamples;
+    if (a.nsamples > 1) {
+      double acc = 0.0;
+      for (int k = 0; k < a.nsamples; ++k) acc += ts[k];
+      tau = acc / a.nsamples;
+    } else {
+      tau = ts[0];
+    }
+  }
+  const double nll = cdm_neg_loglik<L, RICIAN>(a.g, s_cfp, mv, mp, tau, fx, lane);
+  // nll is -loglik (Rician) or (M/2) log(SSE/2) (Gaussian): loglik = -nll.
+  const int np = cdm_np + (RICIAN ? 1 : 0);
+  if (lane == 0) {
+    a.aic[v] = (float)(2.0 * nll + np * 2.0);
+    a.bic[v] = (float)(2.0 * nll + np * log((double)a.g.nmeas));
+  }
+}

@@ -1,0 +1,154 @@
+// Host-side launchers of one model's kernels (included in the model
+// namespace after engine.h).  Grid: ceil(nvox / (256/L)) blocks of 256
+// threads; dynamic LDS holds the staged CFP table plus padding.
+
+static cdm_consts op_consts(const cdm::Consts& in) {
+  cdm_consts c;
+  for (int i = 0; i < cdm_np; ++i) {
+    c.bound_types[i] = in.bound_types[i];
+    c.bmin[i] = in.bmin[i];
+    c.bmax[i] = in.bmax[i];
+    c.prior_types[i] = in.prior_types[i];
+    c.pa[i] = in.pa[i];
+    c.pb[i] = in.pb[i];
+    c.fixed[i] = in.fixed[i];
+  }
+  return c;
+}
+
+static cdm_geom op_geom(const cdm::Geom& g) {
+  cdm_geom k;
+  k.nvox = g.nvox;
+  k.nmeas = g.nmeas;
+  k.cfp_ts = g.cfp_ts;
+  k.fixp_ts = g.fixp_ts;
+  k.meas = g.meas;
+  k.cfp = g.cfp;
+  k.fixp = g.fixp;
+  return k;
+}
+
+static size_t op_lds(const cdm::Geom& g) { return (size_t)(g.nmeas * g.cfp_ts + cdm_cfp_max) * sizeof(float); }
+
+static dim3 op_grid(int nvox, int L) {
+  const int per_block = cdm_block / L;
+  return dim3((unsigned)((nvox + per_block - 1) / per_block));
+}
+
+#define CDM_DISPATCH_L(L, CALL)            \
+  switch (L) {                             \
+    case 1: { constexpr int LG = 1; CALL; } break;   \
+    case 4: { constexpr int LG = 4; CALL; } break;   \
+    case 16: { constexpr int LG = 16; CALL; } break; \
+    case 64: { constexpr int LG = 64; CALL; } break; \
+    default: return hipErrorInvalidValue;  \
+  }
+
+static hipError_t op_gridsearch(int L, const cdm::Geom& g, float* params, int ngp, const int* gparams, int ncombs,
+                                const float* grid_f, const double* grid_d, const int* comb_id, const cdm::Consts& c,
+                                int* kept, hipStream_t s) {
+  if (g.nvox <= 0) return hipSuccess;
+  if (ngp > cdm_np || g.cfp_ts > cdm_cfp_max || g.fixp_ts > cdm_fixp_max) return hipErrorInvalidValue;
+  cdm_gs_args a;
+  a.g = op_geom(g);
+  a.params = params;
+  a.ngp = ngp;
+  for (int i = 0; i < cdm_np; ++i) a.gparams[i] = i < ngp ? gparams[i] : -1;
+  a.ncombs = ncombs;
+  a.grid_f = grid_f;
+  a.grid_d = grid_d;
+  a.comb_id = comb_id;
+  a.c = op_consts(c);
+  a.kept_out = kept;
+  CDM_DISPATCH_L(L, hipLaunchKernelGGL(cdm_gridsearch_kernel<LG>, op_grid(g.nvox, LG), dim3(cdm_block), op_lds(g),
+                                       s, a));
+  return hipGetLastError();
+}
+
+static hipError_t op_levmar(int L, const cdm::Geom& g, float* params, int max_iters, int marquardt,
+                            const cdm::Consts& c, int* iters, float* cost, hipStream_t s) {
+  if (g.nvox <= 0) return hipSuccess;
+  if (g.cfp_ts > cdm_cfp_max || g.fixp_ts > cdm_fixp_max) return hipErrorInvalidValue;
+  cdm_lm_args a;
+  a.g = op_geom(g);
+  a.params = params;
+  a.max_iters = max_iters;
+  a.c = op_consts(c);
+  a.iters_out = iters;
+  a.cost_out = cost;
+  CDM_DISPATCH_L(L, hipLaunchKernelGGL(cdm_levmar_kernel<LG>, op_grid(g.nvox, LG), dim3(cdm_block), op_lds(g), s, a,
+                                       marquardt));
+  return hipGetLastError();
+}
+
+static hipError_t op_mcmc(int L, int rician, const cdm::Geom& g, const cdm::McmcLaunch& m, const cdm::Consts& c,
+                          hipStream_t s) {
+  if (g.nvox <= 0) return hipSuccess;
+  if (g.cfp_ts > cdm_cfp_max || g.fixp_ts > cdm_fixp_max) return hipErrorInvalidValue;
+  cdm_mcmc_args a;
+  a.g = op_geom(g);
+  a.params = m.params;
+  a.state = (cdm_chain*)m.state;
+  a.samples = m.samples;
+  a.tau_samples = m.tau_samples;
+  a.nsamples = m.nsamples;
+  a.sampleevery = m.sampleevery;
+  a.upe = m.upe;
+  a.recording = m.recording;
+  a.update_prop = m.update_prop;
+  a.begin = m.begin;
+  a.count = m.count;
+  a.phase_len = m.phase_len;
+  a.jump_offset = m.jump_offset;
+  a.seed = m.seed;
+  a.voxel_offset = m.voxel_offset;
+  a.c = op_consts(c);
+  if (rician) {
+    CDM_DISPATCH_L(L, hipLaunchKernelGGL((cdm_mcmc_kernel<LG, true>), op_grid(g.nvox, LG), dim3(cdm_block),
+                                         op_lds(g), s, a));
+  } else {
+    CDM_DISPATCH_L(L, hipLaunchKernelGGL((cdm_mcmc_kernel<LG, false>), op_grid(g.nvox, LG), dim3(cdm_block),
+                                         op_lds(g), s, a));
+  }
+  return hipGetLastError();
+}
+
+static hipError_t op_predicted(int L, const cdm::Geom& g, int nsamples, const float* samples, float* out,
+                               hipStream_t s) {
+  if (g.nvox <= 0) return hipSuccess;
+  if (g.cfp_ts > cdm_cfp_max || g.fixp_ts > cdm_fixp_max) return hipErrorInvalidValue;
+  cdm_post_args a;
+  a.g = op_geom(g);
+  a.nsamples = nsamples;
+  a.samples = samples;
+  a.tau_samples = nullptr;
+  a.out = out;
+  a.bic = a.aic = nullptr;
+  CDM_DISPATCH_L(L, hipLaunchKernelGGL(cdm_predicted_kernel<LG>, op_grid(g.nvox, LG), dim3(cdm_block), op_lds(g), s,
+                                       a));
+  return hipGetLastError();
+}
+
+static hipError_t op_bic_aic(int L, int rician, const cdm::Geom& g, int nsamples, const float* samples,
+                             const float* tau_samples, float* bic, float* aic, hipStream_t s) {
+  if (g.nvox <= 0) return hipSuccess;
+  if (g.cfp_ts > cdm_cfp_max || g.fixp_ts > cdm_fixp_max) return hipErrorInvalidValue;
+  cdm_post_args a;
+  a.g = op_geom(g);
+  a.nsamples = nsamples;
+  a.samples = samples;
+  a.tau_samples = tau_samples;
+  a.out = nullptr;
+  a.bic = bic;
+  a.aic = aic;
+  if (rician) {
+    CDM_DISPATCH_L(L, hipLaunchKernelGGL((cdm_bic_aic_kernel<LG, true>), op_grid(g.nvox, LG), dim3(cdm_block),
+                                         op_lds(g), s, a));
+  } else {
+    CDM_DISPATCH_L(L, hipLaunchKernelGGL((cdm_bic_aic_kernel<LG, false>), op_grid(g.nvox, LG), dim3(cdm_block),
+                                         op_lds(g), s, a));
+  }
+  return hipGetLastError();
+}
+
+#undef CDM_DISPATCH_L

@@ -1,0 +1,457 @@
+// C-ABI of the fitting engine (declarations and reference citations in
+// include/cudimot_amd.h).  Host-side state per model handle: the parsed
+// specification, device copies of the search grid, the MCMC chain-state
+// buffer and the launch choices (voxel-group size per method, MCMC blocks).
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/cudimot_amd.h"
+#include "model_ops.h"
+#include "spec.h"
+
+struct cdm_model {
+  const cdm::ModelOps* ops = nullptr;
+  cdm::ModelSpec spec;
+  int group[5] = {0, 0, 0, 0, 0};
+  int mcmc_block = 0;
+  int mcmc_graph = 0;
+  // device grid (valid combinations only)
+  bool grid_dirty = true;
+  int ncombs_valid = 0;
+  float* d_grid_f = nullptr;
+  double* d_grid_d = nullptr;
+  int* d_comb_id = nullptr;
+  // MCMC chain state
+  void* d_state = nullptr;
+  size_t state_bytes = 0;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string& msg) {
+  g_err = msg;
+  return -1;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return -2;
+}
+
+std::string lib_dir() {
+  Dl_info info;
+  if (dladdr((void*)&cdm_version, &info) && info.dli_fname) {
+    std::string p(info.dli_fname);
+    const size_t s = p.rfind('/');
+    return s == std::string::npos ? std::string(".") : p.substr(0, s);
+  }
+  return ".";
+}
+
+cdm::Consts make_consts(const cdm_model* m) {
+  cdm::Consts c;
+  std::memset(&c, 0, sizeof(c));
+  const cdm::ModelSpec& s = m->spec;
+  for (int i = 0; i < s.nparams && i < cdm::kMaxParams; ++i) {
+    c.bound_types[i] = s.bound_types[i];
+    c.bmin[i] = (float)s.bmin[i];
+    c.bmax[i] = (float)s.bmax[i];
+    c.prior_types[i] = s.prior_types[i];
+    c.pa[i] = (float)s.pa[i];
+    c.pb[i] = (float)s.pb[i];
+    c.fixed[i] = s.fixed[i];
+  }
+  return c;
+}
+
+int tsize(int n, const int* sizes) {
+  int t = 0;
+  for (int i = 0; i < n; ++i) t += sizes[i];
+  return t;
+}
+
+bool valid_group(int L) { return L == 1 || L == 4 || L == 16 || L == 64; }
+
+// Built-in voxel-group choice: more lanes per voxel when a voxel has many
+// measurements, so each lane keeps a handful of them.
+int default_group(int method, int nmeas) {
+  (void)method;
+  if (nmeas >= 192) return 16;
+  if (nmeas >= 48) return 4;
+  return 4;
+}
+
+int group_for(const cdm_model* m, int method, int nmeas) {
+  const int g = m->group[method];
+  return g ? g : default_group(method, nmeas);
+}
+
+cdm::Geom make_geom(const cdm_model* m, int nvox, int nmeas, const float* meas, const float* cfp, const float* fixp) {
+  cdm::Geom g;
+  g.nvox = nvox;
+  g.nmeas = nmeas;
+  g.cfp_ts = tsize(m->ops->ncfp, m->ops->cfp_sizes);
+  g.fixp_ts = tsize(m->ops->nfixp, m->ops->fixp_sizes);
+  g.meas = meas;
+  g.cfp = cfp;
+  g.fixp = fixp;
+  return g;
+}
+
+int check_geom(const cdm_model* m, int nvox, int nmeas, const void* meas, const void* cfp, const void* fixp,
+               bool need_meas) {
+  if (!m || !m->ops) return fail("null model handle");
+  if (nvox < 0 || nmeas <= 0) return fail("the number of voxels must be >= 0 and of measurements > 0");
+  if (need_meas && nvox > 0 && !meas) return fail("measurements pointer is null");
+  const int cts = tsize(m->ops->ncfp, m->ops->cfp_sizes), fts = tsize(m->ops->nfixp, m->ops->fixp_sizes);
+  if (cts > 0 && !cfp) return fail("this model needs common fixed parameters (--CFP) but cfp is null");
+  if (fts > 0 && nvox > 0 && !fixp) return fail("this model needs per-voxel fixed parameters (--FixP) but fixp is null");
+  if (cts > m->ops->cfp_max) return fail("CFP rows per measurement exceed the engine limit");
+  if (fts > m->ops->fixp_max) return fail("FixP values per voxel exceed the engine limit");
+  return 0;
+}
+
+void free_grid(cdm_model* m) {
+  if (m->d_grid_f) (void)hipFree(m->d_grid_f);
+  if (m->d_grid_d) (void)hipFree(m->d_grid_d);
+  if (m->d_comb_id) (void)hipFree(m->d_comb_id);
+  m->d_grid_f = nullptr;
+  m->d_grid_d = nullptr;
+  m->d_comb_id = nullptr;
+}
+
+// Combinations whose grid values pass the float bounds test of
+// GridSearch.cu:25-47; the non-grid parameters are tested per voxel on the
+// device.
+int upload_grid(cdm_model* m) {
+  if (!m->grid_dirty) return 0;
+  free_grid(m);
+  const cdm::ModelSpec& s = m->spec;
+  const int ngp = (int)s.grid_params.size();
+  std::vector<float> gf;
+  std::vector<double> gd;
+  std::vector<int> id;
+  for (int c = 0; c < s.grid_combs; ++c) {
+    bool ok = true;
+    for (int k = 0; k < ngp; ++k) {
+      const int p = s.grid_params[k];
+      const double v = s.grid[(size_t)c * ngp + k];
+      const float lo = (float)s.bmin[p], hi = (float)s.bmax[p];
+      const int bt = s.bound_types[p];
+      if ((bt == cdm::BMIN || bt == cdm::BMINMAX) && v < lo) ok = false;
+      if ((bt == cdm::BMAX || bt == cdm::BMINMAX) && v > hi) ok = false;
+    }
+    if (!ok) continue;
+    for (int k = 0; k < ngp; ++k) {
+      gd.push_back(s.grid[(size_t)c * ngp + k]);
+      gf.push_back((float)s.grid[(size_t)c * ngp + k]);
+    }
+    id.push_back(c);
+  }
+  m->ncombs_valid = (int)id.size();
+  if (m->ncombs_valid > 0) {
+    hipError_t e;
+    if ((e = hipMalloc((void**)&m->d_grid_f, gf.size() * sizeof(float))) != hipSuccess) return hip_fail(e, "grid");
+    if ((e = hipMalloc((void**)&m->d_grid_d, gd.size() * sizeof(double))) != hipSuccess) return hip_fail(e, "grid");
+    if ((e = hipMalloc((void**)&m->d_comb_id, id.size() * sizeof(int))) != hipSuccess) return hip_fail(e, "grid");
+    if ((e = hipMemcpy(m->d_grid_f, gf.data(), gf.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
+      return hip_fail(e, "grid upload");
+    if ((e = hipMemcpy(m->d_grid_d, gd.data(), gd.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+      return hip_fail(e, "grid upload");
+    if ((e = hipMemcpy(m->d_comb_id, id.data(), id.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+      return hip_fail(e, "grid upload");
+  }
+  m->grid_dirty = false;
+  return 0;
+}
+
+int ensure_state(cdm_model* m, size_t bytes) {
+  if (m->state_bytes >= bytes) return 0;
+  if (m->d_state) (void)hipFree(m->d_state);
+  m->d_state = nullptr;
+  m->state_bytes = 0;
+  hipError_t e = hipMalloc(&m->d_state, bytes);
+  if (e != hipSuccess) return hip_fail(e, "MCMC chain state");
+  m->state_bytes = bytes;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cdm_version(void) { return 100; }
+
+const char* cdm_last_error(void) { return g_err.c_str(); }
+
+int cdm_list_models(char* buf, int buflen) {
+  std::string s;
+  for (int i = 0; i < cdm::model_count(); ++i) {
+    if (i) s += ",";
+    s += cdm::model_at(i)->name;
+  }
+  if (buf && buflen > 0) {
+    std::strncpy(buf, s.c_str(), (size_t)buflen - 1);
+    buf[buflen - 1] = 0;
+  }
+  return (int)s.size();
+}
+
+int cdm_model_open(const char* name, const char* priors_file, cdm_model** out) {
+  if (!out) return fail("null output handle");
+  *out = nullptr;
+  if (!name) return fail("null model name");
+  const cdm::ModelOps* ops = cdm::find_model(name);
+  if (!ops) return fail(std::string("model not compiled into this library: ") + name);
+  cdm_model* m = new cdm_model();
+  m->ops = ops;
+  const std::string path =
+      priors_file ? std::string(priors_file) : lib_dir() + "/../mymodels/" + std::string(name) + "/modelpriors";
+  const std::string err = cdm::parse_priors_file(path, ops->nparams, m->spec);
+  if (!err.empty()) {
+    delete m;
+    return fail(err);
+  }
+  *out = m;
+  return 0;
+}
+
+void cdm_model_close(cdm_model* m) {
+  if (!m) return;
+  free_grid(m);
+  if (m->d_state) (void)hipFree(m->d_state);
+  delete m;
+}
+
+int cdm_model_info(const cdm_model* m, int* nparams, int* cfp_tsize, int* fixp_tsize) {
+  if (!m) return fail("null model handle");
+  if (nparams) *nparams = m->ops->nparams;
+  if (cfp_tsize) *cfp_tsize = tsize(m->ops->ncfp, m->ops->cfp_sizes);
+  if (fixp_tsize) *fixp_tsize = tsize(m->ops->nfixp, m->ops->fixp_sizes);
+  return 0;
+}
+
+int cdm_model_spec(const cdm_model* m, int* bound_types, double* bmin, double* bmax, int* prior_types, double* pa,
+                   double* pb, int* fixed, double* p_init, int* init_provided) {
+  if (!m) return fail("null model handle");
+  const cdm::ModelSpec& s = m->spec;
+  for (int i = 0; i < s.nparams; ++i) {
+    if (bound_types) bound_types[i] = s.bound_types[i];
+    if (bmin) bmin[i] = s.bmin[i];
+    if (bmax) bmax[i] = s.bmax[i];
+    if (prior_types) prior_types[i] = s.prior_types[i];
+    if (pa) pa[i] = s.pa[i];
+    if (pb) pb[i] = s.pb[i];
+    if (fixed) fixed[i] = s.fixed[i];
+    if (p_init) p_init[i] = s.init_provided ? s.p_init[i] : 0.0;
+  }
+  if (init_provided) *init_provided = s.init_provided ? 1 : 0;
+  return 0;
+}
+
+int cdm_model_set_fixed(cdm_model* m, const char* list) {
+  if (!m) return fail("null model handle");
+  const std::string err = cdm::parse_fixed(list ? list : "", m->spec);
+  return err.empty() ? 0 : fail(err);
+}
+
+int cdm_model_load_grid(cdm_model* m, const char* src, int is_text, int* ncombs, int* ngrid_params) {
+  if (!m) return fail("null model handle");
+  if (!src) return fail("null grid source");
+  const std::string err = is_text ? cdm::parse_grid_text(src, m->spec) : cdm::parse_grid_file(src, m->spec);
+  if (!err.empty()) return fail(err);
+  m->grid_dirty = true;
+  if (ncombs) *ncombs = m->spec.grid_combs;
+  if (ngrid_params) *ngrid_params = (int)m->spec.grid_params.size();
+  return 0;
+}
+
+int cdm_model_grid(const cdm_model* m, int* grid_params, double* grid) {
+  if (!m) return fail("null model handle");
+  if (grid_params) std::copy(m->spec.grid_params.begin(), m->spec.grid_params.end(), grid_params);
+  if (grid) std::copy(m->spec.grid.begin(), m->spec.grid.end(), grid);
+  return 0;
+}
+
+int cdm_model_set_group(cdm_model* m, int method, int lanes) {
+  if (!m) return fail("null model handle");
+  if (method < 0 || method > 4) return fail("unknown method");
+  if (lanes != 0 && !valid_group(lanes)) return fail("voxel-group size must be 1, 4, 16 or 64 lanes");
+  m->group[method] = lanes;
+  return 0;
+}
+
+int cdm_model_get_group(const cdm_model* m, int method, int nmeas) {
+  if (!m || method < 0 || method > 4) return fail("bad arguments");
+  return group_for(m, method, nmeas);
+}
+
+int cdm_model_set_mcmc_blocks(cdm_model* m, int jumps_per_launch, int use_graph) {
+  if (!m) return fail("null model handle");
+  if (jumps_per_launch < 0) return fail("jumps per launch must be >= 0");
+  m->mcmc_block = jumps_per_launch;
+  m->mcmc_graph = use_graph ? 1 : 0;
+  return 0;
+}
+
+int cdm_gridsearch(cdm_model* m, int nvox, int nmeas, const float* meas, float* params, const float* cfp,
+                   const float* fixp, int* kept, void* stream) {
+  int r = check_geom(m, nvox, nmeas, meas, cfp, fixp, true);
+  if (r) return r;
+  if (m->spec.grid_combs <= 0) return fail("GridSearch needs a grid (--gridSearch file with search[i]=(...) lines)");
+  if ((r = upload_grid(m))) return r;
+  if (nvox == 0) return 0;
+  const cdm::Geom g = make_geom(m, nvox, nmeas, meas, cfp, fixp);
+  const cdm::Consts c = make_consts(m);
+  const hipError_t e = m->ops->gridsearch(group_for(m, cdm::kGridSearch, nmeas), g, params,
+                                          (int)m->spec.grid_params.size(), m->spec.grid_params.data(),
+                                          m->ncombs_valid, m->d_grid_f, m->d_grid_d, m->d_comb_id, c, kept,
+                                          (hipStream_t)stream);
+  return e == hipSuccess ? 0 : hip_fail(e, "GridSearch launch");
+}
+
+int cdm_levmar(cdm_model* m, int nvox, int nmeas, const float* meas, float* params, const float* cfp,
+               const float* fixp, int max_iters, int marquardt, int* iters, float* cost, void* stream) {
+  int r = check_geom(m, nvox, nmeas, meas, cfp, fixp, true);
+  if (r) return r;
+  if (max_iters < 0) return fail("--iterLevMar must be >= 0");
+  if (nvox == 0) return 0;
+  const cdm::Geom g = make_geom(m, nvox, nmeas, meas, cfp, fixp);
+  const hipError_t e = m->ops->levmar(group_for(m, cdm::kLevMar, nmeas), g, params, max_iters, marquardt ? 1 : 0,
+                                      make_consts(m), iters, cost, (hipStream_t)stream);
+  return e == hipSuccess ? 0 : hip_fail(e, "Levenberg-Marquardt launch");
+}
+
+int cdm_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params, const float* cfp,
+             const float* fixp, int nburn, int njumps, int sampleevery, int update_proposal, int rician,
+             unsigned long long seed, long long voxel_offset, float* samples, float* tau_samples, void* stream) {
+  int r = check_geom(m, nvox, nmeas, meas, cfp, fixp, true);
+  if (r) return r;
+  if (sampleevery <= 0) return fail("--sampleevery must be > 0");
+  if (njumps < sampleevery)
+    return fail("If MCMC is run, the method must record at least one sample. The number of jumps (--nj) must be "
+                "greater than or equal to space between samples (--se)");
+  if (!samples && nvox > 0) return fail("samples pointer is null");
+  if (rician && !tau_samples && nvox > 0) return fail("--rician needs tau_samples");
+  if (nvox == 0) return 0;
+  if (njumps % sampleevery != 0)
+    return fail("--nj must be a multiple of --se (the reference writes past the sample buffer otherwise)");
+  const int nb = nburn < 1 ? 1 : nburn;  // MCMC.cu:702-703
+  const int nsamples = njumps / sampleevery;
+  if ((r = ensure_state(m, m->ops->chain_bytes * (size_t)nvox))) return r;
+  const cdm::Geom g = make_geom(m, nvox, nmeas, meas, cfp, fixp);
+  const cdm::Consts c = make_consts(m);
+  const int L = group_for(m, cdm::kMcmc, nmeas);
+  hipStream_t s = (hipStream_t)stream;
+
+  cdm::McmcLaunch ml;
+  ml.params = params;
+  ml.state = m->d_state;
+  ml.samples = samples;
+  ml.tau_samples = tau_samples;
+  ml.nsamples = nsamples;
+  ml.sampleevery = sampleevery;
+  ml.upe = sampleevery;
+  ml.seed = seed;
+  ml.voxel_offset = voxel_offset;
+  for (int phase = 0; phase < 2; ++phase) {
+    const int len = phase == 0 ? nb : njumps;
+    const int blk = m->mcmc_block > 0 ? m->mcmc_block : len;
+    ml.recording = phase;
+    ml.update_prop = phase == 0 ? 1 : (update_proposal ? 1 : 0);
+    ml.phase_len = len;
+    ml.jump_offset = phase == 0 ? 0u : (unsigned)nb;
+    hipGraph_t graph = nullptr;
+    const bool use_graph = m->mcmc_graph && len > blk;
+    if (use_graph) {
+      hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) return hip_fail(e, "MCMC graph capture");
+    }
+    for (int b = 0; b < len; b += blk) {
+      ml.begin = b;
+      ml.count = std::min(blk, len - b);
+      const hipError_t e = m->ops->mcmc(L, rician ? 1 : 0, g, ml, c, s);
+      if (e != hipSuccess) {
+        if (use_graph) {
+          hipGraph_t dummy;
+          (void)hipStreamEndCapture(s, &dummy);
+          if (dummy) (void)hipGraphDestroy(dummy);
+        }
+        return hip_fail(e, "MCMC launch");
+      }
+    }
+    if (use_graph) {
+      hipError_t e = hipStreamEndCapture(s, &graph);
+      if (e != hipSuccess) return hip_fail(e, "MCMC graph capture");
+      hipGraphExec_t exec = nullptr;
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (e == hipSuccess) e = hipGraphLaunch(exec, s);
+      if (exec) (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) return hip_fail(e, "MCMC graph launch");
+    }
+  }
+  return 0;
+}
+
+int cdm_predicted_signal(cdm_model* m, int nvox, int nmeas, int nsamples, const float* samples, const float* cfp,
+                         const float* fixp, float* out, void* stream) {
+  int r = check_geom(m, nvox, nmeas, nullptr, cfp, fixp, false);
+  if (r) return r;
+  if (nsamples <= 0) return fail("nsamples must be > 0");
+  if (nvox == 0) return 0;
+  const cdm::Geom g = make_geom(m, nvox, nmeas, nullptr, cfp, fixp);
+  const hipError_t e =
+      m->ops->predicted(group_for(m, cdm::kPredicted, nmeas), g, nsamples, samples, out, (hipStream_t)stream);
+  return e == hipSuccess ? 0 : hip_fail(e, "predicted-signal launch");
+}
+
+int cdm_bic_aic(cdm_model* m, int nvox, int nmeas, int nsamples, int rician, const float* meas, const float* samples,
+                const float* tau_samples, const float* cfp, const float* fixp, float* bic, float* aic, void* stream) {
+  int r = check_geom(m, nvox, nmeas, meas, cfp, fixp, true);
+  if (r) return r;
+  if (nsamples <= 0) return fail("nsamples must be > 0");
+  if (rician && !tau_samples && nvox > 0) return fail("Rician BIC/AIC needs tau samples");
+  if (nvox == 0) return 0;
+  const cdm::Geom g = make_geom(m, nvox, nmeas, meas, cfp, fixp);
+  const hipError_t e = m->ops->bic_aic(group_for(m, cdm::kBicAic, nmeas), rician ? 1 : 0, g, nsamples, samples,
+                                       tau_samples, bic, aic, (hipStream_t)stream);
+  return e == hipSuccess ? 0 : hip_fail(e, "BIC/AIC launch");
+}
+
+int cdm_rician_clean(int nvox, int nmeas, float* meas) {
+  if (nvox < 0 || nmeas <= 0 || (!meas && nvox > 0)) return fail("bad arguments");
+  for (int v = 0; v < nvox; ++v) {
+    float* x = meas + (size_t)v * nmeas;
+    float mn = x[0], mx = x[0];
+    for (int k = 1; k < nmeas; ++k) {
+      mn = std::min(mn, x[k]);
+      mx = std::max(mx, x[k]);
+    }
+    if (mn <= 0 && mx > 0) {
+      float minpos = mx;
+      for (int k = 0; k < nmeas; ++k)
+        if (x[k] > 0 && x[k] < minpos) minpos = x[k];
+      for (int k = 0; k < nmeas; ++k)
+        if (x[k] <= 0) x[k] = minpos;
+    }
+  }
+  return 0;
+}
+
+int cdm_sync(void* stream) {
+  hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "device synchronise");
+  e = hipGetLastError();
+  return e == hipSuccess ? 0 : hip_fail(e, "device error");
+}
+
+}  // extern "C"

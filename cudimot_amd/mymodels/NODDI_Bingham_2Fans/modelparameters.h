@@ -1,0 +1,15 @@
+#ifndef MODEL_H_INCLUDED
+#define MODEL_H_INCLUDED
+// Parameters: fiso, fFan2, fintra1, kappa1, beta1, th1, ph1, psi1, fintra2, kappa2, beta2, th2, ph2, psi2.  CFP: bvecs (3), bvals (1).  FixP: S0.
+// MyType selects the precision the engine evaluates this model's forward
+// signal and Jacobian in (float: FP32 on the GPU).
+typedef float MyType;
+#define NPARAMS 14
+#define NCFP 2
+#define NFIXP 1
+
+struct MODEL {
+  static int CFP_size[NCFP];
+  static int FixP_size[NFIXP];
+};
+#endif

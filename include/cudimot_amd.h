@@ -1,0 +1,126 @@
+/* cudimot_amd.h -- C-ABI of the MI355X voxelwise fitting engine.
+ *
+ * Drop-in boundary for cudimot's fitting path.  Each entry point replaces one
+ * method object of the reference's C++ engine (constructor + run()), takes
+ * plain device pointers and sizes, returns 0 on success or a negative status
+ * with a message in cdm_last_error(), and never exits the process (the
+ * reference prints and calls exit(-1): checkcudacalls.h:19-26, Model.cu:97).
+ *
+ * Buffers (device memory, caller-owned, FP32, voxel-major as the reference):
+ *   meas     [nvox][nmeas]            dMRI_Data.cu:137 (meas_host[vox*nmeas+m])
+ *   params   [nvox][nparams]          Parameters.cu:72,81 (in/out)
+ *   cfp      [nmeas][cfp_tsize]       Parameters.cu:119 (CFP_host[i+j*CFP_Tsize])
+ *   fixp     [nvox][fixp_tsize]       Parameters.cu:186
+ *   samples  [nvox][nparams][nsamp]   MCMC.cu:392,621
+ *   tau      [nvox][nsamp]            MCMC.cu:624
+ * `stream` is a hipStream_t (NULL: the null stream); calls are asynchronous
+ * on it.  A model handle is not safe for concurrent use from several host
+ * threads.
+ */
+#ifndef CUDIMOT_AMD_H
+#define CUDIMOT_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cdm_model cdm_model;
+
+enum cdm_method { CDM_GRIDSEARCH = 0, CDM_LEVMAR = 1, CDM_MCMC = 2, CDM_PREDICTED = 3, CDM_BIC_AIC = 4 };
+
+/* Library version (major*10000 + minor*100 + patch). */
+int cdm_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* cdm_last_error(void);
+
+/* Comma-separated names of the model plug-ins compiled into this library.
+ * Returns the full length needed (excluding the terminator). */
+int cdm_list_models(char* buf, int buflen);
+
+/* Open a compiled plug-in and parse its modelpriors file.
+ * Replaces Model<T>::Model + Modelparser (Model.cu:16-299, 395-403); the
+ * reference selects the model at compile time (Makefile:3-5, one binary per
+ * model) and reads <binary>_priors or --priors (Model.cu:20-25).
+ * priors_file: path, or NULL for the plug-in's own modelpriors. */
+int cdm_model_open(const char* name, const char* priors_file, cdm_model** out);
+void cdm_model_close(cdm_model* m);
+
+/* NPARAMS and the summed CFP / FixP row sizes (Model.cu:248-260). */
+int cdm_model_info(const cdm_model* m, int* nparams, int* cfp_tsize, int* fixp_tsize);
+
+/* Parsed specification (arrays of nparams; any pointer may be NULL).
+ * Bounds and prior arguments are the parsed doubles; the kernels use them
+ * rounded to float, as the reference's __constant__ float copies
+ * (GridSearch.cu:271-280, Levenberg_Marquardt.cu:643-660, MCMC.cu:723-750). */
+int cdm_model_spec(const cdm_model* m, int* bound_types, double* bmin, double* bmax, int* prior_types,
+                   double* prior_a, double* prior_b, int* fixed, double* p_init, int* init_provided);
+
+/* --fixed=i,j,... (Model.cu:279-298). */
+int cdm_model_set_fixed(cdm_model* m, const char* list);
+
+/* --gridSearch file (Model.cu:302-393).  is_text != 0: `src` is the file
+ * text itself.  Outputs the number of combinations and grid parameters. */
+int cdm_model_load_grid(cdm_model* m, const char* src, int is_text, int* ncombs, int* ngrid_params);
+/* Grid read-back: grid_params [ngrid_params], grid [ncombs][ngrid_params]. */
+int cdm_model_grid(const cdm_model* m, int* grid_params, double* grid);
+
+/* Voxel-group size (lanes per voxel: 1, 4, 16 or 64) used by a method;
+ * 0 restores the built-in choice.  The autotuner sets these. */
+int cdm_model_set_group(cdm_model* m, int method, int lanes);
+int cdm_model_get_group(const cdm_model* m, int method, int nmeas);
+
+/* MCMC jump blocks: split each MCMC phase into launches of at most
+ * `jumps_per_launch` jumps (0: one launch per phase); use_graph != 0
+ * records the launches of a phase into a HIP graph and replays it. */
+int cdm_model_set_mcmc_blocks(cdm_model* m, int jumps_per_launch, int use_graph);
+
+/* GridSearch<T>::run (GridSearch.cu:291-317; kernel :97-248).  Updates the
+ * grid parameters of `params` to the combination of least squared error
+ * (the FP64 argmin, earliest combination on ties).  kept (nullable,
+ * device int[nvox]) receives the kept combination index, -1 if none. */
+int cdm_gridsearch(cdm_model* m, int nvox, int nmeas, const float* meas, float* params, const float* cfp,
+                   const float* fixp, int* kept, void* stream);
+
+/* Levenberg_Marquardt<T>::run (Levenberg_Marquardt.cu:665-704; kernel
+ * :409-623).  max_iters = --iterLevMar (successful iterations);
+ * marquardt = !--no_Marquardt.  iters (int[nvox]) and cost (float[nvox])
+ * are optional device outputs. */
+int cdm_levmar(cdm_model* m, int nvox, int nmeas, const float* meas, float* params, const float* cfp,
+               const float* fixp, int max_iters, int marquardt, int* iters, float* cost, void* stream);
+
+/* MCMC<T>::MCMC + run (MCMC.cu:690-848): burn-in (nburn jumps, at least 1)
+ * then recording (njumps, a sample every sampleevery jumps into samples;
+ * nsamples = njumps / sampleevery).  Proposal widths adapt every
+ * sampleevery jumps (the reference ignores --upe, MCMC.cu:711).
+ * rician: --rician (tau_samples required).  seed/voxel_offset address the
+ * Philox streams: draws depend only on (seed, voxel_offset + voxel, jump). */
+int cdm_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params, const float* cfp,
+             const float* fixp, int nburn, int njumps, int sampleevery, int update_proposal, int rician,
+             unsigned long long seed, long long voxel_offset, float* samples, float* tau_samples, void* stream);
+
+/* getPredictedSignal<T>::run (getPredictedSignal.cu:98-118): signal of the
+ * mean sample, out [nvox][nmeas]. */
+int cdm_predicted_signal(cdm_model* m, int nvox, int nmeas, int nsamples, const float* samples, const float* cfp,
+                         const float* fixp, float* out, void* stream);
+
+/* BIC_AIC<T>::run (BIC_AIC.cu:212-243): BIC and AIC of the mean sample,
+ * Rician form when rician != 0 (the reference uses it only with --rician
+ * and --runMCMC, BIC_AIC.cu:208). */
+int cdm_bic_aic(cdm_model* m, int nvox, int nmeas, int nsamples, int rician, const float* meas,
+                const float* samples, const float* tau_samples, const float* cfp, const float* fixp, float* bic,
+                float* aic, void* stream);
+
+/* Host-side data preparation for --rician: per voxel, replace non-positive
+ * signals by the smallest positive one when some are positive
+ * (dMRI_Data.cu:14-31).  meas is HOST memory [nvox][nmeas]. */
+int cdm_rician_clean(int nvox, int nmeas, float* meas);
+
+/* Block until all work queued on `stream` has finished; reports any
+ * asynchronous device error (the reference's sync_check). */
+int cdm_sync(void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUDIMOT_AMD_H */

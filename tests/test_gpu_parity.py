@@ -1,0 +1,226 @@
+"""GPU parity: the HIP fitting path (through the C-ABI) against the FP64 CPU
+oracle (oracle/oracle_engine.h), model by model, on seeded synthetic volumes.
+
+Tolerances (stated per check):
+  * forward signal (FP32 on GPU vs FP64): max |rel| <= 2e-5 of the voxel's S0 scale
+  * GridSearch: kept combination index identical for every voxel
+  * Levenberg-Marquardt: final cost within 1e-3 relative for >= 97% of voxels,
+    and S0/diffusivity/fraction estimates within 1e-3 relative for >= 90%
+  * MCMC: draw-for-draw identical samples (<= 1e-4 rel) for >= 95% of voxels
+    over the first jumps; posterior means within 0.35 posterior SD (median
+    over voxels) for long chains
+  * BIC/AIC: within 1e-4 relative
+"""
+import numpy as np
+import pytest
+
+from oracle import bind
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+SIMPLE = ["Ball_1_Stick", "Ball_2_Sticks", "Ball_3_Sticks", "DiffusionTensor", "toymodel"]
+
+
+def _models():
+    from cudimot_amd import capi
+    return [m for m in capi.list_models()]
+
+
+def _spec_arrays(spec):
+    return {k: np.asarray(v) for k, v in spec.items() if k not in ("init_provided",)}
+
+
+def _start(model, spec, noisy, cfp, truth):
+    """LM start: P_init, with S0 (when a parameter) set from the b=0 mean."""
+    p0 = np.tile(np.asarray(spec["p_init"], dtype=np.float64), (noisy.shape[0], 1))
+    if model in ("Ball_1_Stick", "Ball_2_Sticks", "Ball_3_Sticks", "DiffusionTensor"):
+        b0 = cfp[:, 3] == 0
+        p0[:, 0] = noisy[:, b0].mean(1)
+    return p0
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available()
+    return torch
+
+
+@pytest.mark.parametrize("model", SIMPLE)
+@pytest.mark.parametrize("lanes", [1, 4, 16, 64])
+def test_predicted_signal_parity(torch_cuda, model, lanes):
+    from cudimot_amd import capi
+    o = bind.OracleModel(model)
+    m = capi.Model(model)
+    m.set_group(capi.PREDICTED, lanes)
+    n = 777
+    cfp = H.ACQ[H.MODEL_ACQ[model]]()
+    t = H.truth(model, n, seed=5)
+    fx = H.fixp_for(model, n)
+    ref = o.signal(t, cfp, fx)
+    out = torch_cuda.zeros((n, cfp.shape[0]), dtype=torch_cuda.float32, device="cuda")
+    samples = H.dev(t.reshape(n, -1, 1))
+    m.predicted_signal(samples, H.dev(cfp), H.dev(fx), out)
+    capi.sync()
+    got = H.host(out)
+    scale = np.abs(ref).max(1, keepdims=True)
+    err = np.abs(got - ref) / scale
+    assert err.max() <= 2e-5, (model, lanes, err.max())
+
+
+GRIDS = {
+    "Ball_1_Stick": "search[1]=(0.0005,0.001,0.002,0.003,0.01)\nsearch[2]=(0.1,0.3,0.5,0.7,0.9)\n"
+                    "search[3]=(0.2,0.8,1.4,2.0,2.6)\nsearch[4]=(-2.5,-1.5,-0.5,0.5,1.5,2.5)\n",
+    "Ball_2_Sticks": "search[2]=(0.2,0.4,0.6)\nsearch[3]=(0.3,1.2,2.1)\nsearch[4]=(-2,0,2)\n"
+                     "search[5]=(0.0,0.1,0.2)\nsearch[6]=(0.6,1.5,2.4)\nsearch[7]=(-1,1,3)\n",
+    "DiffusionTensor": "search[1]=(0.0005,0.0015,0.0025)\nsearch[4]=(0.0005,0.0015,0.0025)\n"
+                       "search[6]=(0.0005,0.0015,0.0025)\nsearch[2]=(-0.0003,0,0.0003)\n",
+    "toymodel": "search[0]=(0.5,1,1.5,2)\nsearch[1]=(0.25,0.5,1,1.5,2)\n",
+}
+
+
+@pytest.mark.parametrize("model", sorted(GRIDS))
+@pytest.mark.parametrize("lanes", [4, 16])
+def test_gridsearch_argmin_exact(torch_cuda, model, lanes):
+    from cudimot_amd import capi
+    o = bind.OracleModel(model)
+    m = capi.Model(model)
+    m.set_group(capi.GRIDSEARCH, lanes)
+    gp, grid = m.load_grid(GRIDS[model], is_text=True)
+    spec = m.spec()
+    n = 600 if model != "Ball_2_Sticks" else 200
+    cfp, t, fx, noisy = H.volume(o, model, n, seed=21)
+    p0 = _start(model, spec, noisy, cfp, t)
+    # the oracle is handed float-rounded data, as the GPU sees it
+    noisy32 = noisy.astype(np.float32).astype(np.float64)
+    p0_32 = p0.astype(np.float32).astype(np.float64)
+    cfp32 = cfp.astype(np.float32).astype(np.float64)
+    p_ref, kept_ref = o.gridsearch(_spec_arrays(spec), noisy32, p0_32, cfp32, fx, np.asarray(gp), np.asarray(grid))
+    dp = H.dev(p0)
+    kept = torch_cuda.full((n,), -7, dtype=torch_cuda.int32, device="cuda")
+    m.gridsearch(H.dev(noisy), dp, H.dev(cfp), H.dev(fx), kept=kept)
+    capi.sync()
+    k = kept.cpu().numpy()
+    mism = np.nonzero(k != kept_ref)[0]
+    assert mism.size == 0, (model, lanes, mism[:10], k[mism[:10]], kept_ref[mism[:10]])
+    np.testing.assert_allclose(H.host(dp), p_ref, rtol=1e-6, atol=1e-7)
+
+
+@pytest.mark.parametrize("model", SIMPLE)
+@pytest.mark.parametrize("lanes", [1, 4, 16, 64])
+def test_levmar_parity(torch_cuda, model, lanes):
+    from cudimot_amd import capi
+    o = bind.OracleModel(model)
+    m = capi.Model(model)
+    m.set_group(capi.LEVMAR, lanes)
+    spec = m.spec()
+    n = 1024 if model not in ("Ball_2_Sticks", "Ball_3_Sticks") else 256
+    cfp, t, fx, noisy = H.volume(o, model, n, seed=31)
+    p0 = _start(model, spec, noisy, cfp, t)
+    noisy32 = noisy.astype(np.float32).astype(np.float64)
+    cfp32 = cfp.astype(np.float32).astype(np.float64)
+    p0_32 = p0.astype(np.float32).astype(np.float64)
+    p_ref, it_ref, c_ref = o.levmar(_spec_arrays(spec), noisy32, p0_32, cfp32, fx, max_iters=200)
+    dp = H.dev(p0)
+    cost = torch_cuda.zeros((n,), dtype=torch_cuda.float32, device="cuda")
+    iters = torch_cuda.zeros((n,), dtype=torch_cuda.int32, device="cuda")
+    m.levmar(H.dev(noisy), dp, H.dev(cfp), H.dev(fx), max_iters=200, iters=iters, cost=cost)
+    capi.sync()
+    p = H.host(dp)
+    c = H.host(cost)
+    # cost at the GPU solution, evaluated by the FP64 oracle
+    c_at = ((o.signal(p, cfp32, fx) - noisy32) ** 2).sum(1)
+    rel = np.abs(c_at - c_ref) / np.maximum(c_ref, 1e-30)
+    frac_cost = np.mean(rel <= 1e-3)
+    assert frac_cost >= 0.97, (model, lanes, frac_cost, np.sort(rel)[-10:])
+    # GPU-reported cost agrees with the oracle's evaluation of it
+    assert np.median(np.abs(c - c_at) / np.maximum(c_at, 1e-30)) < 1e-4
+    idx = {"Ball_1_Stick": [0, 1, 2], "Ball_2_Sticks": [0, 1], "Ball_3_Sticks": [0, 1],
+           "DiffusionTensor": [0, 1, 4, 6], "toymodel": [0, 1]}[model]
+    prel = np.abs(p[:, idx] - p_ref[:, idx]) / np.maximum(np.abs(p_ref[:, idx]), 1e-12)
+    frac_p = np.mean(np.all(prel <= 1e-3, axis=1))
+    assert frac_p >= 0.90, (model, lanes, frac_p)
+
+
+@pytest.mark.parametrize("model", ["Ball_1_Stick", "Ball_2_Sticks", "toymodel"])
+@pytest.mark.parametrize("rician", [False, True])
+def test_mcmc_draw_for_draw(torch_cuda, model, rician):
+    from cudimot_amd import capi
+    o = bind.OracleModel(model)
+    m = capi.Model(model)
+    spec = m.spec()
+    n = 512
+    cfp, t, fx, noisy = H.volume(o, model, n, seed=41)
+    start = t.copy()
+    noisy32 = noisy.astype(np.float32).astype(np.float64)
+    cfp32 = cfp.astype(np.float32).astype(np.float64)
+    start32 = start.astype(np.float32).astype(np.float64)
+    nb, nj, se = 3, 4, 1
+    _, s_ref, tau_ref = o.mcmc(_spec_arrays(spec), noisy32, start32, cfp32, fx, nburn=nb, njumps=nj,
+                               sampleevery=se, rician=rician, seed=99)
+    P = m.nparams
+    samples = torch_cuda.zeros((n, P, nj // se), dtype=torch_cuda.float32, device="cuda")
+    tau = torch_cuda.zeros((n, nj // se), dtype=torch_cuda.float32, device="cuda") if rician else None
+    m.mcmc(H.dev(noisy), H.dev(start), H.dev(cfp), H.dev(fx), samples, tau, nburn=nb, njumps=nj, sampleevery=se,
+           rician=rician, seed=99)
+    capi.sync()
+    s = H.host(samples)
+    rel = np.abs(s - s_ref) / np.maximum(np.abs(s_ref), 1e-6)
+    same = np.all(rel <= 1e-4, axis=(1, 2))
+    assert same.mean() >= 0.95, (model, rician, same.mean())
+
+
+@pytest.mark.parametrize("model", ["Ball_1_Stick", "DiffusionTensor"])
+def test_mcmc_posterior_moments(torch_cuda, model):
+    from cudimot_amd import capi
+    o = bind.OracleModel(model)
+    m = capi.Model(model)
+    spec = m.spec()
+    n = 256
+    cfp, t, fx, noisy = H.volume(o, model, n, seed=51)
+    noisy32 = noisy.astype(np.float32).astype(np.float64)
+    cfp32 = cfp.astype(np.float32).astype(np.float64)
+    start32 = t.astype(np.float32).astype(np.float64)
+    nb, nj, se = 400, 2000, 10
+    _, s_ref, _ = o.mcmc(_spec_arrays(spec), noisy32, start32, cfp32, fx, nburn=nb, njumps=nj, sampleevery=se,
+                         seed=7)
+    samples = torch_cuda.zeros((n, m.nparams, nj // se), dtype=torch_cuda.float32, device="cuda")
+    m.mcmc(H.dev(noisy), H.dev(t), H.dev(cfp), H.dev(fx), samples, None, nburn=nb, njumps=nj, sampleevery=se,
+           seed=7)
+    capi.sync()
+    s = H.host(samples)
+    idx = [0, 1, 2] if model == "Ball_1_Stick" else [0, 1, 4, 6]
+    mu_g, mu_o = s[:, idx].mean(2), s_ref[:, idx].mean(2)
+    sd_o = s_ref[:, idx].std(2) + 1e-30
+    z = np.abs(mu_g - mu_o) / sd_o
+    assert np.median(z) <= 0.35, (model, np.median(z, 0))
+    sd_ratio = s[:, idx].std(2) / sd_o
+    assert 0.8 <= np.median(sd_ratio) <= 1.25, np.median(sd_ratio, 0)
+
+
+@pytest.mark.parametrize("model", SIMPLE)
+@pytest.mark.parametrize("rician", [False, True])
+def test_bic_aic_parity(torch_cuda, model, rician):
+    from cudimot_amd import capi
+    o = bind.OracleModel(model)
+    m = capi.Model(model)
+    n = 300
+    cfp, t, fx, noisy = H.volume(o, model, n, seed=61)
+    noisy32 = noisy.astype(np.float32).astype(np.float64)
+    cfp32 = cfp.astype(np.float32).astype(np.float64)
+    rng = np.random.default_rng(3)
+    ns = 5
+    samples = (t[:, :, None] * (1 + 0.01 * rng.normal(size=(n, t.shape[1], ns)))).astype(np.float32)
+    tau = (1.0 / (noisy.std(1) ** 2))[:, None] * np.ones((1, ns))
+    tau = tau.astype(np.float32)
+    b_ref, a_ref = o.bic_aic(noisy32, samples.astype(np.float64), cfp32, fx, rician=rician,
+                             tau_samples=tau.astype(np.float64))
+    bic = torch_cuda.zeros((n,), dtype=torch_cuda.float32, device="cuda")
+    aic = torch_cuda.zeros((n,), dtype=torch_cuda.float32, device="cuda")
+    m.bic_aic(H.dev(noisy), H.dev(samples), H.dev(cfp), H.dev(fx), bic, aic, rician=rician,
+              tau_samples=H.dev(tau) if rician else None)
+    capi.sync()
+    for got, ref in ((H.host(bic), b_ref), (H.host(aic), a_ref)):
+        err = np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)
+        assert err.max() <= 1e-4, (model, rician, err.max())
