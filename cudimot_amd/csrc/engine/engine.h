@@ -19,6 +19,16 @@
 // chain and LM state are FP64.
 
 typedef MyType cdm_tf;
+// Precision of the Levenberg-Marquardt Jacobian pass.  A plug-in defines
+// CDM_LM_JACOBIAN_DOUBLE when central differences of its single-precision
+// forward model are noise-dominated along near-degenerate directions (the
+// fanning angle of Bingham-NODDI as beta -> 0, crossing sticks started
+// together); MCMC and the cost keep their own precision.
+#ifdef CDM_LM_JACOBIAN_DOUBLE
+typedef double cdm_jt;
+#else
+typedef cdm_tf cdm_jt;
+#endif
 constexpr int cdm_np = NPARAMS;
 constexpr int cdm_ntri = NPARAMS * (NPARAMS + 1) / 2;
 constexpr int cdm_cfp_max = 16;
@@ -261,9 +271,13 @@ __global__ void __launch_bounds__(cdm_block) cdm_gridsearch_kernel(cdm_gs_args a
 // ------------------------------------------------------ Levenberg-Marquardt
 // Levenberg_Marquardt.cu:409-623, with the bound transforms of :37-135 in
 // FP64 against float bounds.  The gradient and Hessian come from one fused
-// pass (the reference makes two passes over the same values); each lane
-// accumulates its measurements in FP32, group sums are FP32, and the damped
-// normal equations are solved by FP64 Cholesky.  A failed factorisation
+// pass in the plug-in's precision (the reference makes two passes over the
+// same values); each lane accumulates its measurements in FP32, group sums
+// are FP32, and the damped normal equations are solved by FP64 Cholesky.
+// The cost that decides acceptance is evaluated with the plug-in
+// instantiated in double: an FP32 cost carries ~5e-7 relative rounding
+// noise, which stops the iteration early in flat valleys (e.g. the kappa-beta
+// ridge of Bingham-NODDI) where the reference keeps improving.  A failed factorisation
 // (non-positive pivot) yields a NaN step, which the cost test rejects, as
 // the reference's zero pivot does.  Only successful iterations count toward
 // max_iters; the loop also ends on a relative cost change <= 1e-8 or
@@ -369,10 +383,10 @@ __global__ void __launch_bounds__(cdm_block) cdm_levmar_kernel(cdm_lm_args a, in
   for (int i = 0; i < cdm_np; ++i) p[i] = a.params[(size_t)v * cdm_np + i];
   cdm_inv_transform(a.c, p, t);
 
-  cdm_tf pf[cdm_np];
+  double fxd[cdm_fixp_max];
 #pragma unroll
-  for (int i = 0; i < cdm_np; ++i) pf[i] = (cdm_tf)p[i];
-  double pcf = cdm_sse<L>(a.g, s_cfp, mv, pf, fx, lane);
+  for (int k = 0; k < cdm_fixp_max; ++k) fxd[k] = (double)fx[k];
+  double pcf = cdm_sse<L>(a.g, s_cfp, mv, p, fxd, lane);
 
   double lambda = 0.1, olambda = 0.0;
   bool success = true;
@@ -380,27 +394,32 @@ __global__ void __launch_bounds__(cdm_block) cdm_levmar_kernel(cdm_lm_args a, in
   float hs[cdm_ntri];   // 2 J^T J, packed lower triangle (undamped)
   double grad[cdm_np];  // 2 J^T r
   double diag[cdm_np];  // current damped diagonal
-  cdm_cfp_row<cdm_tf> row;
 
   while (!(success && iter++ >= a.max_iters)) {
     if (success) {
       double dt[cdm_np];
       cdm_dtransform(a.c, t, dt);
-      cdm_tf dtf[cdm_np];
+      cdm_jt dtf[cdm_np], pj[cdm_np], fxj[cdm_fixp_max];
 #pragma unroll
-      for (int i = 0; i < cdm_np; ++i) dtf[i] = (cdm_tf)dt[i];
+      for (int i = 0; i < cdm_np; ++i) {
+        dtf[i] = (cdm_jt)dt[i];
+        pj[i] = (cdm_jt)p[i];
+      }
+#pragma unroll
+      for (int k = 0; k < cdm_fixp_max; ++k) fxj[k] = (cdm_jt)fxd[k];
       float gl[cdm_np];
 #pragma unroll
       for (int i = 0; i < cdm_np; ++i) gl[i] = 0.f;
 #pragma unroll
       for (int k = 0; k < cdm_ntri; ++k) hs[k] = 0.f;
+      cdm_cfp_row<cdm_jt> rowj;
       for (int m = lane; m < a.g.nmeas; m += L) {
-        cdm_tf* cm = row.at(s_cfp, m, a.g.cfp_ts);
-        const cdm_tf r = Predicted_Signal(cdm_np, pf, cm, fx) - (cdm_tf)mv[m];
-        cdm_tf d[cdm_np];
+        cdm_jt* cm = rowj.at(s_cfp, m, a.g.cfp_ts);
+        const cdm_jt r = Predicted_Signal(cdm_np, pj, cm, fxj) - (cdm_jt)mv[m];
+        cdm_jt d[cdm_np];
 #pragma unroll
-        for (int i = 0; i < cdm_np; ++i) d[i] = (cdm_tf)0;
-        Partial_Derivatives(cdm_np, pf, cm, fx, d);
+        for (int i = 0; i < cdm_np; ++i) d[i] = (cdm_jt)0;
+        Partial_Derivatives(cdm_np, pj, cm, fxj, d);
         float df[cdm_np];
 #pragma unroll
         for (int i = 0; i < cdm_np; ++i) df[i] = (float)(d[i] * dtf[i]);
@@ -483,9 +502,7 @@ __global__ void __launch_bounds__(cdm_block) cdm_levmar_kernel(cdm_lm_args a, in
 #pragma unroll
     for (int i = 0; i < cdm_np; ++i) tn[i] = spd ? t[i] - step[i] : NAN;
     cdm_transform(a.c, p, tn);
-#pragma unroll
-    for (int i = 0; i < cdm_np; ++i) pf[i] = (cdm_tf)p[i];
-    const double ncf = cdm_sse<L>(a.g, s_cfp, mv, pf, fx, lane);
+    const double ncf = cdm_sse<L>(a.g, s_cfp, mv, p, fxd, lane);
     bool end = false;
     success = ncf < pcf;
     if (success) {
@@ -501,8 +518,6 @@ __global__ void __launch_bounds__(cdm_block) cdm_levmar_kernel(cdm_lm_args a, in
       lambda = lambda * 10.0;
       if (lambda > cdm_lm_ltol) end = true;
       cdm_transform(a.c, p, t);
-#pragma unroll
-      for (int i = 0; i < cdm_np; ++i) pf[i] = (cdm_tf)p[i];
     }
     if (end) break;
   }

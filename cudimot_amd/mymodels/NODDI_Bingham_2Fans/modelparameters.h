@@ -6,6 +6,9 @@
 typedef float MyType;
 #define NPARAMS 14
 #define NCFP 2
+// Levenberg-Marquardt Jacobian in double (engine option): near-degenerate
+// directions make single-precision central differences noise-dominated.
+#define CDM_LM_JACOBIAN_DOUBLE
 #define NFIXP 1
 
 struct MODEL {

@@ -1,0 +1,78 @@
+"""The C-ABI library loads without a GPU, exports every entry point that
+include/cudimot_amd.h declares, and its host-only calls work on the CPU."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from cudimot_amd import capi
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "cudimot_amd.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(cdm_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(capi.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 18
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_models_compiled_in():
+    assert set(capi.list_models()) >= {"Ball_1_Stick", "Ball_2_Sticks", "Ball_3_Sticks", "DiffusionTensor",
+                                        "NODDI_Watson", "NODDI_Bingham", "NODDI_Bingham_2Fans", "toymodel"}
+
+
+def test_unknown_model_is_an_error():
+    with pytest.raises(capi.CdmError, match="not compiled"):
+        capi.Model("NoSuchModel")
+
+
+def test_model_info_sizes():
+    m = capi.Model("NODDI_Bingham")
+    assert (m.nparams, m.cfp_tsize, m.fixp_tsize) == (7, 4, 1)
+    m = capi.Model("toymodel")
+    assert (m.nparams, m.cfp_tsize, m.fixp_tsize) == (2, 1, 0)
+
+
+def test_group_size_validation():
+    m = capi.Model("Ball_1_Stick")
+    m.set_group(capi.LEVMAR, 16)
+    assert m.get_group(capi.LEVMAR, 64) == 16
+    m.set_group(capi.LEVMAR, 0)
+    assert m.get_group(capi.LEVMAR, 64) in (1, 4, 16, 64)
+    with pytest.raises(capi.CdmError):
+        m.set_group(capi.LEVMAR, 3)
+
+
+def test_argument_errors_without_gpu():
+    """Errors found on the host come back as status + message, no exit."""
+    m = capi.Model("Ball_1_Stick")
+    L = capi.lib()
+    assert L.cdm_levmar(m.h, 10, 64, None, None, None, None, 100, 1, None, None, None) != 0
+    assert "null" in L.cdm_last_error().decode()
+    assert L.cdm_mcmc(m.h, 10, 64, 8, 8, 8, None, 10, 10, 25, 1, 0, 1, 0, 8, None, None) != 0
+    assert "at least one sample" in L.cdm_last_error().decode()
+    assert L.cdm_gridsearch(m.h, 10, 64, 8, 8, 8, None, None, None) != 0
+    assert "grid" in L.cdm_last_error().decode()
+    n = capi.Model("NODDI_Watson")
+    assert L.cdm_levmar(n.h, 10, 99, 8, 8, 8, None, 100, 1, None, None, None) != 0
+    assert "FixP" in L.cdm_last_error().decode()
+
+
+def test_rician_clean_matches_reference_rule():
+    """dMRI_Data.cu:14-31: non-positive entries take the smallest positive
+    value when some entries are positive; all-non-positive voxels stay."""
+    x = np.array([[5.0, -1.0, 0.0, 3.0, 4.0], [0.0, -2.0, 0.0, -1.0, 0.0], [1.0, 2.0, 3.0, 4.0, 5.0]],
+                 dtype=np.float32)
+    capi.rician_clean(x)
+    np.testing.assert_array_equal(x[0], [5, 3, 3, 3, 4])
+    np.testing.assert_array_equal(x[1], [0, -2, 0, -1, 0])
+    np.testing.assert_array_equal(x[2], [1, 2, 3, 4, 5])
