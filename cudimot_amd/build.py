@@ -117,6 +117,44 @@ def build(models=None, extra_dirs=(), force=False, jobs=None):
     return LIB
 
 
+OPC_LIB = os.path.join(LIBDIR, "libcdm_opcount.so")
+
+
+def build_opcount(models=None):
+    """Host library of the FLOP model: every plug-in compiled with an
+    operation-counting scalar (csrc/tools/opcount.h)."""
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    template = open(os.path.join(CSRC, "tools", "opcount_unit.cpp.in")).read()
+    objs = []
+    jobs_list = []
+    for name in (models or shipped_models()):
+        d = os.path.join(MODELS, name)
+        src = os.path.join(BUILD, "opc_%s.cpp" % name)
+        text = template.replace("@NAME@", name).replace("@DIR@", d)
+        if not os.path.exists(src) or open(src).read() != text:
+            with open(src, "w") as f:
+                f.write(text)
+        jobs_list.append((src, os.path.join(BUILD, "opc_%s.o" % name), [d, os.path.join(CSRC, "tools"),
+                                                                          os.path.join(CSRC, "abi")]))
+
+    def comp(src, obj, deps):
+        stamp = obj + ".sha"
+        key = _deps_hash([src] + list(deps))
+        if not (os.path.exists(obj) and os.path.exists(stamp) and open(stamp).read() == key):
+            _run(["g++", "-O1", "-std=c++17", "-fPIC", "-w", "-I", CSRC, "-c", src, "-o", obj])
+            with open(stamp, "w") as f:
+                f.write(key)
+        return obj
+
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
+        objs = list(ex.map(lambda j: comp(*j), jobs_list))
+    tmp = OPC_LIB + ".tmp"
+    _run(["g++", "-shared", "-o", tmp] + objs)
+    os.replace(tmp, OPC_LIB)
+    return OPC_LIB
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--models", default="", help="comma-separated shipped plug-ins (default: all)")
@@ -125,6 +163,7 @@ def main(argv=None):
     a = ap.parse_args(argv)
     models = [m for m in a.models.split(",") if m] or None
     print(build(models, a.extra_model_dir, a.force))
+    print(build_opcount(models))
 
 
 if __name__ == "__main__":

@@ -1,0 +1,64 @@
+"""Algorithmic work model of the fitting methods (FLOPs by precision).
+
+Per forward evaluation F_fwd and per Partial_Derivatives call F_jac are
+counted (not estimated) by running each plug-in on the CPU with an
+operation-counting scalar (lib/libcdm_opcount.so, csrc/tools/opcount.h):
++, -, * = 1, / and sqrt = 1, each transcendental = 1, FMA = 2.  Method
+totals per voxel:
+  MCMC  = jumps * P * M * (F_fwd + 3 FP32 + 1 FP64)   (residual, square,
+          FP64 accumulation per measurement; proposals and priors are O(P)
+          per jump and not counted)
+  LM    = sum over iterations of M * (F_fwd + F_jac + P(P+1)/2 * 2 + 2P)
+          for accepted steps plus M * F_fwd per cost evaluation
+  GS    = combinations * M * (F_fwd + 3)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libcdm_opcount.so")
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError("FLOP model not built: %s (python -m cudimot_amd.build)" % LIB)
+        _lib = ctypes.CDLL(LIB)
+    return _lib
+
+
+def per_eval(model, params, cfp, fixp=None):
+    """Average counts per (voxel, measurement): dict with fwd/jac FP32/FP64
+    flops and transcendental calls."""
+    params = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+    cfp = np.ascontiguousarray(cfp, dtype=np.float64)
+    n = params.shape[0]
+    fixp = np.ascontiguousarray(fixp if fixp is not None else np.zeros((n, 1)), dtype=np.float64)
+    out = np.zeros(8)
+    D = ctypes.POINTER(ctypes.c_double)
+    getattr(_load(), "cdm_opcount_" + model)(n, cfp.shape[0], params.ctypes.data_as(D), cfp.ctypes.data_as(D),
+                                             fixp.ctypes.data_as(D), out.ctypes.data_as(D))
+    return {"fwd32": out[0], "fwd64": out[1], "fwd_t32": out[2], "fwd_t64": out[3],
+            "jac32": out[4], "jac64": out[5], "jac_t32": out[6], "jac_t64": out[7]}
+
+
+def mcmc_flops(ev, nparams, nmeas, jumps):
+    """(FP32, FP64) flops of one voxel's MCMC over `jumps` jumps."""
+    n = jumps * nparams * nmeas
+    return n * (ev["fwd32"] + 3), n * (ev["fwd64"] + 1)
+
+
+def lm_flops(ev, nparams, nmeas, iters, jac_double=False):
+    """(FP32, FP64) flops of one voxel's LM with `iters` accepted iterations
+    (rejected steps cost one FP64 cost evaluation; counted as accepted ones
+    in the cost term, a lower bound on the Jacobian term)."""
+    tri = nparams * (nparams + 1) // 2
+    jac32 = 0 if jac_double else ev["fwd32"] + ev["jac32"]
+    jac64 = (ev["fwd32"] + ev["jac32"]) if jac_double else ev["fwd64"] + ev["jac64"]
+    per_acc32 = nmeas * (jac32 + 2 * tri + 2 * nparams)
+    per_acc64 = nmeas * jac64
+    cost64 = nmeas * (ev["fwd32"] + ev["fwd64"] + 3)  # FP64 cost evaluation
+    return iters * per_acc32, iters * per_acc64 + (iters + 1) * cost64

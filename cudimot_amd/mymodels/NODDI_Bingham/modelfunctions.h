@@ -9,7 +9,7 @@
 
 MACRO T Predicted_Signal(int npar, T* P, T* CFP, T* FixP) {
   bingham_fan f;
-  bingham_fan_setup((double)P[1], (double)P[2], (double)P[3], f);
+  bingham_fan_setup((CDM_F64)P[1], (CDM_F64)P[2], (CDM_F64)P[3], f);
   const T fan = bingham_fan_signal(P[1], P[2], P[3], P[4], P[5], P[6], f, CFP);
   const T iso = exp_gpu(-CFP[3] * (T)CDM_NODDI_DISO);
   return FixP[0] * (((T)1 - P[0]) * fan + P[0] * iso);
@@ -20,7 +20,7 @@ MACRO bool ConstraintsMCMC(int npar, T* P) { return !((P[2] - (T)0.05) < P[3]); 
 
 MACRO void Partial_Derivatives(int npar, T* P, T* CFP, T* FixP, T* derivatives) {
   bingham_fan f;
-  bingham_fan_setup((double)P[1], (double)P[2], (double)P[3], f);
+  bingham_fan_setup((CDM_F64)P[1], (CDM_F64)P[2], (CDM_F64)P[3], f);
   const T fan = bingham_fan_signal(P[1], P[2], P[3], P[4], P[5], P[6], f, CFP);
   // compartments for the fintra derivative: fan = (1-fintra) E + fintra I
   const T e = bingham_fan_signal((T)0, P[2], P[3], P[4], P[5], P[6], f, CFP);

@@ -9,8 +9,8 @@
 
 MACRO T Predicted_Signal(int npar, T* P, T* CFP, T* FixP) {
   bingham_fan f1, f2;
-  bingham_fan_setup((double)P[2], (double)P[3], (double)P[4], f1);
-  bingham_fan_setup((double)P[8], (double)P[9], (double)P[10], f2);
+  bingham_fan_setup((CDM_F64)P[2], (CDM_F64)P[3], (CDM_F64)P[4], f1);
+  bingham_fan_setup((CDM_F64)P[8], (CDM_F64)P[9], (CDM_F64)P[10], f2);
   const T fan1 = bingham_fan_signal(P[2], P[3], P[4], P[5], P[6], P[7], f1, CFP);
   const T fan2 = bingham_fan_signal(P[8], P[9], P[10], P[11], P[12], P[13], f2, CFP);
   const T aniso = ((T)1 - P[1]) * fan1 + P[1] * fan2;

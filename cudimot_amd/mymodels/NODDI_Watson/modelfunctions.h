@@ -15,17 +15,17 @@ MACRO void watson_compartments(T* P, T* CFP, T& extra, T& intra) {
   tsincos(P[4], &sp, &cp);
   T gn = CFP[0] * st * cp + CFP[1] * st * sp + CFP[2] * ct;
   // per-voxel quantities in double
-  double dpar_eq, dperp_eq, coef[7], lgi[7];
-  noddi_watson_hindered((double)P[2], CDM_NODDI_DPAR * (1.0 - (double)P[1]), dpar_eq, dperp_eq);
-  noddi_watson_sh((double)P[2], coef);
-  noddi_lgi((double)CFP[3] * CDM_NODDI_DPAR, lgi);
+  CDM_F64 dpar_eq, dperp_eq, coef[7], lgi[7];
+  noddi_watson_hindered((CDM_F64)P[2], CDM_NODDI_DPAR * (1.0 - (CDM_F64)P[1]), dpar_eq, dperp_eq);
+  noddi_watson_sh((CDM_F64)P[2], coef);
+  noddi_lgi((CDM_F64)CFP[3] * CDM_NODDI_DPAR, lgi);
   extra = exp_gpu(-CFP[3] * ((T)(dpar_eq - dperp_eq) * gn * gn + (T)dperp_eq));
   if (gn > (T)1) gn = (T)1;
   if (gn < (T)-1) gn = (T)-1;
   T sh[7];
   noddi_even_sh(gn, sh);
-  double s = 0.0;
-  for (int l = 0; l < 7; ++l) s += lgi[l] * coef[l] * (double)sh[l];
+  CDM_F64 s = 0.0;
+  for (int l = 0; l < 7; ++l) s += lgi[l] * coef[l] * (CDM_F64)sh[l];
   if (s < 0) s = 1e-3;
   intra = (T)(0.5 * s);
 }

@@ -5,6 +5,11 @@
 // NODDI_Watson/LegendreFunctions.h:6-107); the per-voxel quantities are
 // evaluated in double whatever the plug-in's precision.
 
+// Scalar type of the double-precision parts (an operation-counting type
+// when the plug-in is compiled for the FLOP model).
+#ifndef CDM_F64
+#define CDM_F64 double
+#endif
 #define CDM_NODDI_DISO 3.0e-3  // isotropic diffusivity (mm^2/s), diffusivities.h
 #define CDM_NODDI_DPAR 1.7e-3  // intra-axonal parallel diffusivity (mm^2/s)
 
@@ -17,22 +22,22 @@
 //                                                     + e^{-(2k+1) 2 xp h}/(n0-2k-1)]
 // with xp = |x| - n0 h and w_k = exp(-((2k+1) h)^2).  This is the
 // approximation (and its ~2e-7 accuracy) the reference uses.
-__device__ inline double noddi_dawson(double x) {
-  const double h = 0.4;
-  const double w[6] = {0.8521437889662113, 0.23692775868212165, 0.01831563888873418,
+__device__ inline CDM_F64 noddi_dawson(CDM_F64 x) {
+  const CDM_F64 h = 0.4;
+  const CDM_F64 w[6] = {0.8521437889662113, 0.23692775868212165, 0.01831563888873418,
                        0.0003936690406550776, 2.352575200009771e-06, 3.90893843426485e-09};
   if (fabs(x) < 0.2) {
-    const double x2 = x * x;
+    const CDM_F64 x2 = x * x;
     return x * (1.0 - (2.0 / 3.0) * x2 * (1.0 - 0.4 * x2 * (1.0 - (2.0 / 7.0) * x2)));
   }
-  const double ax = fabs(x);
+  const CDM_F64 ax = fabs(x);
   const int n0 = 2 * (int)(0.5 * ax / h + 0.5);
-  const double xp = ax - n0 * h;
-  double e1 = exp(2.0 * xp * h);
-  const double e2 = e1 * e1;
-  double up = n0 + 1, dn = up - 2.0, acc = 0.0;
+  const CDM_F64 xp = ax - n0 * h;
+  CDM_F64 e1 = exp(2.0 * xp * h);
+  const CDM_F64 e2 = e1 * e1;
+  CDM_F64 up = n0 + 1, dn = up - 2.0, acc = 0.0;
   for (int k = 0; k < 6; ++k, up += 2.0, dn -= 2.0, e1 *= e2) acc += w[k] * (e1 / up + 1.0 / (dn * e1));
-  double r = fabs(exp(-xp * xp));
+  CDM_F64 r = fabs(exp(-xp * xp));
   if (x < 0) r = -r;
   return 0.5641895835 * r * acc;
 }
@@ -41,16 +46,16 @@ __device__ inline double noddi_dawson(double x) {
 // sticks with parallel diffusivity dpar and perpendicular dperp
 // (WatsonFunctions.h:53-89): outputs the apparent parallel and
 // perpendicular diffusivities of the orientation-averaged tensor.
-__device__ inline void noddi_watson_hindered(double kappa, double dperp, double& dpar_eq, double& dperp_eq) {
-  const double dpar = CDM_NODDI_DPAR;
-  const double d = dpar - dperp;
+__device__ inline void noddi_watson_hindered(CDM_F64 kappa, CDM_F64 dperp, CDM_F64& dpar_eq, CDM_F64& dperp_eq) {
+  const CDM_F64 dpar = CDM_NODDI_DPAR;
+  const CDM_F64 d = dpar - dperp;
   if (kappa < 1e-5) {
-    const double s = dpar + 2.0 * dperp, k2 = kappa * kappa;
+    const CDM_F64 s = dpar + 2.0 * dperp, k2 = kappa * kappa;
     dpar_eq = s / 3.0 + 4.0 * d * kappa / 45.0 + 8.0 * d * k2 / 945.0;
     dperp_eq = s / 3.0 - 2.0 * d * kappa / 45.0 - 4.0 * d * k2 / 945.0;
   } else {
-    const double sk = sqrt(kappa);
-    const double ratio = sk / noddi_dawson(sk);
+    const CDM_F64 sk = sqrt(kappa);
+    const CDM_F64 ratio = sk / noddi_dawson(sk);
     dpar_eq = (-d + 2.0 * dperp * kappa + d * ratio) / (2.0 * kappa);
     dperp_eq = (d + 2.0 * (dpar + dperp) * kappa - d * ratio) / (4.0 * kappa);
   }
@@ -60,28 +65,28 @@ __device__ inline void noddi_watson_hindered(double kappa, double dperp, double&
 //   c_l = sqrt(4 pi (4l+1)) * int_0^1 e^{k t^2} P_2l(t) dt / int_0^1 e^{k t^2} dt
 // with the 24-point Gauss-Legendre rule (12 positive nodes, even integrand).
 // Every term is O(1), so the absolute error stays near 1e-16.
-__device__ inline void noddi_watson_sh_quad(double k, double* c) {
-  const double node[12][2] = {
+__device__ inline void noddi_watson_sh_quad(CDM_F64 k, CDM_F64* c) {
+  const CDM_F64 node[12][2] = {
       {0.06405689286260563, 0.12793819534675221}, {0.1911188674736163, 0.1258374563468283},
       {0.3150426796961634, 0.12167047292780342},  {0.4337935076260451, 0.11550566805372561},
       {0.5454214713888396, 0.1074442701159656},   {0.6480936519369755, 0.09761865210411406},
       {0.7401241915785544, 0.08619016153195329},  {0.820001985973903, 0.07334648141108041},
       {0.886415527004401, 0.05929858491543674},   {0.9382745520027328, 0.04427743881741955},
       {0.9747285559713095, 0.028531388628933743}, {0.9951872199970213, 0.012341229799987091}};
-  double acc[7] = {0, 0, 0, 0, 0, 0, 0}, z = 0.0;
+  CDM_F64 acc[7] = {0, 0, 0, 0, 0, 0, 0}, z = 0.0;
   for (int q = 0; q < 12; ++q) {
-    const double t = node[q][0], wq = node[q][1] * exp(k * t * t);
+    const CDM_F64 t = node[q][0], wq = node[q][1] * exp(k * t * t);
     z += wq;
-    double pm = 1.0, p = t;  // Legendre P_0, P_1
+    CDM_F64 pm = 1.0, p = t;  // Legendre P_0, P_1
     acc[0] += wq;
     for (int l = 2; l <= 12; ++l) {
-      const double pn = (t * (2 * l - 1) * p - (l - 1) * pm) / l;
+      const CDM_F64 pn = (t * (2 * l - 1) * p - (l - 1) * pm) / l;
       pm = p;
       p = pn;
       if ((l & 1) == 0) acc[l / 2] += wq * p;
     }
   }
-  const double pi = 3.14159265358979323846;
+  const CDM_F64 pi = 3.14159265358979323846;
   for (int l = 0; l < 7; ++l) c[l] = sqrt(4.0 * pi * (4 * l + 1)) * acc[l] / z;
 }
 
@@ -93,8 +98,8 @@ __device__ inline void noddi_watson_sh_quad(double k, double* c) {
 // 0.4 < kappa <= 3 the closed forms cancel catastrophically (the reference's
 // intra-cellular signal is off by up to ~1% at kappa = 0.41, and by more
 // than 1e-6 up to kappa ~ 2), so that band uses noddi_watson_sh_quad.
-__device__ inline void noddi_watson_sh(double k, double* c) {
-  const double pi = 3.14159265358979323846;
+__device__ inline void noddi_watson_sh(CDM_F64 k, CDM_F64* c) {
+  const CDM_F64 pi = 3.14159265358979323846;
   c[0] = 2.0 * sqrt(pi);
   if (k > 0.4 && k <= 3.0) {
     noddi_watson_sh_quad(k, c);
@@ -102,16 +107,16 @@ __device__ inline void noddi_watson_sh(double k, double* c) {
   }
   if (k > 30) {
     // fits in u = log(kappa) - log(30), coefficients of u^0..u^6
-    const double fit[6][7] = {
+    const CDM_F64 fit[6][7] = {
         {7.52308, 0.411538, -0.214588, 0.0784091, -0.023981, 0.00731537, -0.0026467},
         {8.93718, 1.62147, -0.733421, 0.191568, -0.0202906, -0.00779095, 0.00574847},
         {8.87905, 3.35689, -1.15935, 0.0673053, 0.121857, -0.066642, 0.0180215},
         {7.84352, 5.03178, -1.0193, -0.426362, 0.328816, -0.0688176, -0.0229398},
         {6.30113, 6.09914, -0.16088, -1.05578, 0.338069, 0.0937157, -0.106935},
         {4.65678, 6.30069, 1.13754, -1.38393, -0.0134758, 0.331686, -0.105954}};
-    const double u = log(k) - log(30.0);
+    const CDM_F64 u = log(k) - log(30.0);
     for (int l = 0; l < 6; ++l) {
-      double v = fit[l][0], pw = u;
+      CDM_F64 v = fit[l][0], pw = u;
       for (int j = 1; j < 7; ++j) {
         v = v + fit[l][j] * pw;
         pw = pw * u;
@@ -119,14 +124,14 @@ __device__ inline void noddi_watson_sh(double k, double* c) {
       c[l + 1] = v;
     }
   } else if (k > 0.4) {
-    const double sk = sqrt(k);
-    const double sk2 = sk * k, sk3 = sk2 * k, sk4 = sk3 * k, sk5 = sk4 * k, sk6 = sk5 * k;
-    const double k2 = k * k, k3 = k2 * k, k4 = k3 * k, k5 = k4 * k, k6 = k5 * k;
-    const double dw = noddi_dawson(sk);
-    const double erfik = (2.0 / 1.772453850905516) * exp(sk * sk) * dw;  // erfi(sqrt k)
-    const double ierfik = 1.0 / erfik;
-    const double ek = exp(k);
-    double v;
+    const CDM_F64 sk = sqrt(k);
+    const CDM_F64 sk2 = sk * k, sk3 = sk2 * k, sk4 = sk3 * k, sk5 = sk4 * k, sk6 = sk5 * k;
+    const CDM_F64 k2 = k * k, k3 = k2 * k, k4 = k3 * k, k5 = k4 * k, k6 = k5 * k;
+    const CDM_F64 dw = noddi_dawson(sk);
+    const CDM_F64 erfik = (2.0 / 1.772453850905516) * exp(sk * sk) * dw;  // erfi(sqrt k)
+    const CDM_F64 ierfik = 1.0 / erfik;
+    const CDM_F64 ek = exp(k);
+    CDM_F64 v;
     v = 3.0 * sk - (3.0 + 2.0 * k) * dw;
     v = sqrt(5.0) * v * ek;
     c[1] = v * ierfik / k;
@@ -159,7 +164,7 @@ __device__ inline void noddi_watson_sh(double k, double* c) {
     v = v / 16384.0 / k6;
     c[6] = v * ierfik;
   } else {
-    const double k2 = k * k, k3 = k2 * k, k4 = k3 * k, k5 = k4 * k, k6 = k5 * k;
+    const CDM_F64 k2 = k * k, k3 = k2 * k, k4 = k3 * k, k5 = k4 * k, k6 = k5 * k;
     c[1] = (4.0 / 3.0 * k + 8.0 / 63.0 * k2) * sqrt(pi / 5.0);
     c[2] = (8.0 / 21.0 * k2 + 32.0 / 693.0 * k3) * (sqrt(pi) * 0.2);
     c[3] = (16.0 / 693.0 * k3 + 32.0 / 10395.0 * k4) * sqrt(pi / 13.0);
@@ -178,12 +183,12 @@ __device__ inline void noddi_watson_sh(double k, double* c) {
 // (erff) inside its double path, and the upward recurrence multiplies that
 // error ~60-fold by I_6 at b = 700 s/mm^2, enough to move the intra-cellular
 // signal by up to ~1e-3 at large kappa; here erf is double.
-__device__ inline void noddi_lgi(double x, double* L) {
+__device__ inline void noddi_lgi(CDM_F64 x, CDM_F64* L) {
   if (x > 0.05) {
-    double I[7];
-    const double sx = sqrt(x);
+    CDM_F64 I[7];
+    const CDM_F64 sx = sqrt(x);
     I[0] = sqrt(3.14159265358979323846) * erf(sx) / sx;
-    const double dx = 1.0 / x, emx = -exp(-x);
+    const CDM_F64 dx = 1.0 / x, emx = -exp(-x);
     for (int i = 1; i < 7; ++i) {
       I[i] = emx + ((i + 1) - 1.5) * I[i - 1];
       I[i] = I[i] * dx;
@@ -199,7 +204,7 @@ __device__ inline void noddi_lgi(double x, double* L) {
            (1021020.0 / 1024.0) * I[3] + (2078505.0 / 1024.0) * I[4] - (1939938.0 / 1024.0) * I[5] +
            (676039.0 / 1024.0) * I[6];
   } else {
-    const double x2 = x * x, x3 = x2 * x, x4 = x3 * x, x5 = x4 * x, x6 = x5 * x;
+    const CDM_F64 x2 = x * x, x3 = x2 * x, x4 = x3 * x, x5 = x4 * x, x6 = x5 * x;
     L[0] = 2.0 - 2.0 * x / 3.0 + x2 / 5.0 - x3 / 21.0 + x4 / 108.0;
     L[1] = -4.0 * x / 15.0 + 4.0 * x2 / 35.0 - 2.0 * x3 / 63.0 + 2.0 * x4 / 297.0;
     L[2] = 8.0 * x2 / 315.0 - 8.0 * x3 / 693.0 + 4.0 * x4 / 1287.0;
