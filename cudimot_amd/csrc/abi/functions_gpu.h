@@ -36,6 +36,12 @@ using std::max;
 using std::min;
 #endif
 
+// Scalar type of a plug-in's double-precision parts (an operation-counting
+// type in the FLOP-model build).
+#ifndef CDM_F64
+#define CDM_F64 double
+#endif
+
 CDM_FUNC float log_gpu(float x) { return logf(x); }
 CDM_FUNC double log_gpu(double x) { return log(x); }
 CDM_FUNC float exp_gpu(float x) { return expf(x); }

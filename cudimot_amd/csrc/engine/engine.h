@@ -15,7 +15,7 @@
 //
 // Precision: the plug-in's forward model and Jacobian run in MyType (FP32
 // for the shipped plug-ins); costs and log-likelihoods accumulate in FP64;
-// the LM normal equations are solved by an FP64 Cholesky factorisation;
+// the LM normal equations are solved by an FP64 LDL^T (Cholesky) factorisation;
 // chain and LM state are FP64.
 
 typedef MyType cdm_tf;
@@ -26,11 +26,55 @@ typedef MyType cdm_tf;
 // together); MCMC and the cost keep their own precision.
 #ifdef CDM_LM_JACOBIAN_DOUBLE
 typedef double cdm_jt;
+typedef double cdm_lm_acc;  // J^T J and J^T r sums
 #else
 typedef cdm_tf cdm_jt;
+typedef float cdm_lm_acc;
 #endif
 constexpr int cdm_np = NPARAMS;
 constexpr int cdm_ntri = NPARAMS * (NPARAMS + 1) / 2;
+
+// Optional invariant caches declared by the plug-in (engine extension, no
+// counterpart in the reference's plug-in API):
+//   CDM_VOXEL_CACHE n          n values that depend only on the parameters in
+//   CDM_VOXEL_CACHE_DEPENDS b  bit mask b, computed by Voxel_Cache(npar, P,
+//                              FixP, vc) once per parameter vector
+//   CDM_MEAS_CACHE k           k values per measurement from Meas_Cache(CFP,
+//                              mc), computed once per block into LDS
+//   Predicted_Signal_Cached(npar, P, CFP, FixP, vc, mc)
+//   CDM_NUMERICAL_MASK b       derivatives the engine forms by central
+//                              differences (the reference's NUMERICAL step
+//                              rule) with per-perturbation voxel caches;
+//                              Partial_Derivatives_Cached fills the others
+// Without them the plug-in's Predicted_Signal / Partial_Derivatives are used.
+#ifdef CDM_VOXEL_CACHE
+constexpr int cdm_vc = CDM_VOXEL_CACHE;
+constexpr unsigned cdm_vc_dep = CDM_VOXEL_CACHE_DEPENDS;
+#else
+constexpr int cdm_vc = 1;
+constexpr unsigned cdm_vc_dep = 0;
+#endif
+#ifdef CDM_MEAS_CACHE
+constexpr int cdm_mc = CDM_MEAS_CACHE;
+#else
+constexpr int cdm_mc = 0;
+#endif
+
+template <class TF>
+__device__ __forceinline__ void cdm_prepare(TF* p, TF* fx, CDM_F64* vc) {
+#ifdef CDM_VOXEL_CACHE
+  Voxel_Cache(cdm_np, p, fx, vc);
+#endif
+}
+
+template <class TF>
+__device__ __forceinline__ TF cdm_signal(TF* p, TF* cfp, TF* fx, const CDM_F64* vc, const CDM_F64* mc) {
+#ifdef CDM_VOXEL_CACHE
+  return Predicted_Signal_Cached(cdm_np, p, cfp, fx, vc, mc);
+#else
+  return Predicted_Signal(cdm_np, p, cfp, fx);
+#endif
+}
 constexpr int cdm_cfp_max = 16;
 constexpr int cdm_fixp_max = 8;
 constexpr int cdm_block = 256;
@@ -58,10 +102,26 @@ struct cdm_geom {
 
 // ------------------------------------------------------------------ helpers
 
-__device__ __forceinline__ void cdm_stage_cfp(const cdm_geom& g, float* s_cfp) {
+// LDS layout: CFP table (float, padded), then the measurement cache (double).
+__host__ __device__ __forceinline__ int cdm_mc_offset(int nmeas, int cfp_ts) {
+  return ((nmeas * cfp_ts + cdm_cfp_max + 1) / 2) * 2;  // in floats, 8-byte aligned
+}
+
+__device__ __forceinline__ const CDM_F64* cdm_stage_cfp(const cdm_geom& g, float* s_cfp) {
   const int n = g.nmeas * g.cfp_ts;
   for (int i = threadIdx.x; i < n + cdm_cfp_max; i += blockDim.x) s_cfp[i] = i < n ? g.cfp[i] : 0.f;
+  CDM_F64* s_mc = reinterpret_cast<CDM_F64*>(s_cfp + cdm_mc_offset(g.nmeas, g.cfp_ts));
+#ifdef CDM_MEAS_CACHE
   __syncthreads();
+  for (int m = threadIdx.x; m < g.nmeas; m += blockDim.x)
+    Meas_Cache(const_cast<float*>(&s_cfp[m * g.cfp_ts]), &s_mc[m * cdm_mc]);
+#endif
+  __syncthreads();
+  return s_mc;
+}
+
+__device__ __forceinline__ const CDM_F64* cdm_mc_row(const CDM_F64* s_mc, int m) {
+  return cdm_mc ? s_mc + m * cdm_mc : nullptr;
 }
 
 template <class TF>
@@ -87,14 +147,16 @@ struct cdm_cfp_row<float> {
   }
 };
 
-// Sum of squared residuals for parameters p (converted to TF), FP64 sums.
+// Sum of squared residuals for parameters p (in TF), FP64 sums.
 template <int L, class TF>
-__device__ __forceinline__ double cdm_sse(const cdm_geom& g, const float* s_cfp, const float* mv, TF* p, TF* fx,
-                                          int lane) {
+__device__ __forceinline__ double cdm_sse(const cdm_geom& g, const float* s_cfp, const CDM_F64* s_mc,
+                                          const float* mv, TF* p, TF* fx, int lane) {
   double acc = 0.0;
   cdm_cfp_row<TF> row;
+  CDM_F64 vc[cdm_vc];
+  cdm_prepare(p, fx, vc);
   for (int m = lane; m < g.nmeas; m += L) {
-    const TF s = Predicted_Signal(cdm_np, p, row.at(s_cfp, m, g.cfp_ts), fx);
+    const TF s = cdm_signal(p, row.at(s_cfp, m, g.cfp_ts), fx, vc, cdm_mc_row(s_mc, m));
     const TF r = s - (TF)mv[m];
     acc += (double)(r * r);
   }
@@ -132,7 +194,7 @@ struct cdm_gs_args {
 template <int L>
 __global__ void __launch_bounds__(cdm_block) cdm_gridsearch_kernel(cdm_gs_args a) {
   extern __shared__ float s_cfp[];
-  cdm_stage_cfp(a.g, s_cfp);
+  const CDM_F64* s_mc = cdm_stage_cfp(a.g, s_cfp);
   const int v = blockIdx.x * (cdm_block / L) + threadIdx.x / L;
   const int lane = threadIdx.x % L;
   if (v >= a.g.nvox) return;
@@ -188,7 +250,7 @@ __global__ void __launch_bounds__(cdm_block) cdm_gridsearch_kernel(cdm_gs_args a
       for (int i = 0; i < cdm_np; ++i)
         if (i == a.gparams[k]) trial[i] = (cdm_tf)gv;
     }
-    const float cost = (float)cdm_sse<L>(a.g, s_cfp, mv, trial, fx, lane);
+    const float cost = (float)cdm_sse<L>(a.g, s_cfp, s_mc, mv, trial, fx, lane);
     if (cost < best_c[CDM_GS_TOPK - 1]) {
       // insertion keeps (cost, index) order; equal costs keep the earlier index first
       float cc = cost;
@@ -250,7 +312,7 @@ __global__ void __launch_bounds__(cdm_block) cdm_gridsearch_kernel(cdm_gs_args a
         for (int i = 0; i < cdm_np; ++i)
           if (i == a.gparams[k]) trial_d[i] = gv;
       }
-      const double cd = cdm_sse<L>(a.g, s_cfp, mv, trial_d, fx_d, lane);
+      const double cd = cdm_sse<L>(a.g, s_cfp, s_mc, mv, trial_d, fx_d, lane);
       if (cd < bestd) {
         bestd = cd;
         bestd_i = c;
@@ -271,15 +333,15 @@ __global__ void __launch_bounds__(cdm_block) cdm_gridsearch_kernel(cdm_gs_args a
 // ------------------------------------------------------ Levenberg-Marquardt
 // Levenberg_Marquardt.cu:409-623, with the bound transforms of :37-135 in
 // FP64 against float bounds.  The gradient and Hessian come from one fused
-// pass in the plug-in's precision (the reference makes two passes over the
-// same values); each lane accumulates its measurements in FP32, group sums
-// are FP32, and the damped normal equations are solved by FP64 Cholesky.
-// The cost that decides acceptance is evaluated with the plug-in
-// instantiated in double: an FP32 cost carries ~5e-7 relative rounding
-// noise, which stops the iteration early in flat valleys (e.g. the kappa-beta
-// ridge of Bingham-NODDI) where the reference keeps improving.  A failed factorisation
-// (non-positive pivot) yields a NaN step, which the cost test rejects, as
-// the reference's zero pivot does.  Only successful iterations count toward
+// pass in the Jacobian precision (the reference makes two passes over the
+// same values); each lane accumulates its measurements (in FP32, or FP64
+// with CDM_LM_JACOBIAN_DOUBLE) before the group sums, and the damped normal
+// equations are solved by FP64 LDL^T.  The cost that decides acceptance is
+// evaluated with the plug-in instantiated in double: an FP32 cost carries
+// ~5e-7 relative rounding noise, which stops the iteration early in flat
+// valleys (e.g. the kappa-beta ridge of Bingham-NODDI) where the reference
+// keeps improving.  A zero pivot yields an inf/NaN step, which the cost test
+// rejects, as in the reference.  Only successful iterations count toward
 // max_iters; the loop also ends on a relative cost change <= 1e-8 or
 // lambda > 1e20.
 
@@ -367,10 +429,69 @@ __device__ __forceinline__ void cdm_dtransform(const cdm_consts& c, const double
   }
 }
 
+// Signal and Jacobian at one measurement.  With CDM_NUMERICAL_MASK the
+// engine forms the listed derivatives by the reference's central difference
+// (step clamp(rel |p|, lo, hi), probes p + h/2 and p + h/2 - 2 (h/2)), using
+// voxel caches prepared once per perturbation; otherwise the plug-in's
+// Partial_Derivatives is called.
+template <class TJ>
+struct cdm_jacobian_setup {
+  CDM_F64 vc0[cdm_vc];
+#ifdef CDM_NUMERICAL_MASK
+  CDM_F64 vcp[cdm_np][cdm_vc], vcm[cdm_np][cdm_vc];
+  TJ h[cdm_np];
+#endif
+  __device__ __forceinline__ void init(TJ* p, TJ* fx) {
+    cdm_prepare(p, fx, vc0);
+#ifdef CDM_NUMERICAL_MASK
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) {
+      if (!((CDM_NUMERICAL_MASK >> i) & 1u)) continue;
+      TJ hi = fabs_gpu((TJ)::cdm_fd_step<TJ>::rel * p[i]);
+      hi = min_gpu(max_gpu(hi, (TJ)::cdm_fd_step<TJ>::lo), (TJ)::cdm_fd_step<TJ>::hi);
+      h[i] = hi;
+      if ((cdm_vc_dep >> i) & 1u) {
+        const TJ keep = p[i];
+        p[i] += hi * (TJ)0.5;
+        cdm_prepare(p, fx, vcp[i]);
+        p[i] -= (TJ)2 * (hi * (TJ)0.5);
+        cdm_prepare(p, fx, vcm[i]);
+        p[i] = keep;
+      }
+    }
+#endif
+  }
+  __device__ __forceinline__ TJ eval(TJ* p, TJ* cm, TJ* fx, const CDM_F64* mc, TJ* d) {
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) d[i] = (TJ)0;
+#ifdef CDM_NUMERICAL_MASK
+    const TJ s = cdm_signal(p, cm, fx, vc0, mc);
+    Partial_Derivatives_Cached(cdm_np, p, cm, fx, vc0, mc, d);
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) {
+      if (!((CDM_NUMERICAL_MASK >> i) & 1u)) continue;
+      const bool dep = (cdm_vc_dep >> i) & 1u;
+      const TJ keep = p[i];
+      p[i] += h[i] * (TJ)0.5;
+      const TJ up = cdm_signal(p, cm, fx, dep ? vcp[i] : vc0, mc);
+      p[i] -= (TJ)2 * (h[i] * (TJ)0.5);
+      const TJ dn = cdm_signal(p, cm, fx, dep ? vcm[i] : vc0, mc);
+      p[i] = keep;
+      d[i] = (up - dn) / h[i];
+    }
+    return s;
+#else
+    const TJ s = cdm_signal(p, cm, fx, vc0, mc);
+    Partial_Derivatives(cdm_np, p, cm, fx, d);
+    return s;
+#endif
+  }
+};
+
 template <int L>
 __global__ void __launch_bounds__(cdm_block) cdm_levmar_kernel(cdm_lm_args a, int marquardt) {
   extern __shared__ float s_cfp[];
-  cdm_stage_cfp(a.g, s_cfp);
+  const CDM_F64* s_mc = cdm_stage_cfp(a.g, s_cfp);
   const int v = blockIdx.x * (cdm_block / L) + threadIdx.x / L;
   const int lane = threadIdx.x % L;
   if (v >= a.g.nvox) return;
@@ -386,12 +507,12 @@ __global__ void __launch_bounds__(cdm_block) cdm_levmar_kernel(cdm_lm_args a, in
   double fxd[cdm_fixp_max];
 #pragma unroll
   for (int k = 0; k < cdm_fixp_max; ++k) fxd[k] = (double)fx[k];
-  double pcf = cdm_sse<L>(a.g, s_cfp, mv, p, fxd, lane);
+  double pcf = cdm_sse<L>(a.g, s_cfp, s_mc, mv, p, fxd, lane);
 
   double lambda = 0.1, olambda = 0.0;
   bool success = true;
   int iter = 0, nsucc = 0;
-  float hs[cdm_ntri];   // 2 J^T J, packed lower triangle (undamped)
+  cdm_lm_acc hs[cdm_ntri];  // 2 J^T J, packed lower triangle (undamped)
   double grad[cdm_np];  // 2 J^T r
   double diag[cdm_np];  // current damped diagonal
 
@@ -407,23 +528,22 @@ __global__ void __launch_bounds__(cdm_block) cdm_levmar_kernel(cdm_lm_args a, in
       }
 #pragma unroll
       for (int k = 0; k < cdm_fixp_max; ++k) fxj[k] = (cdm_jt)fxd[k];
-      float gl[cdm_np];
+      cdm_lm_acc gl[cdm_np];
 #pragma unroll
-      for (int i = 0; i < cdm_np; ++i) gl[i] = 0.f;
+      for (int i = 0; i < cdm_np; ++i) gl[i] = 0;
 #pragma unroll
-      for (int k = 0; k < cdm_ntri; ++k) hs[k] = 0.f;
+      for (int k = 0; k < cdm_ntri; ++k) hs[k] = 0;
       cdm_cfp_row<cdm_jt> rowj;
+      cdm_jacobian_setup<cdm_jt> jac;
+      jac.init(pj, fxj);
       for (int m = lane; m < a.g.nmeas; m += L) {
         cdm_jt* cm = rowj.at(s_cfp, m, a.g.cfp_ts);
-        const cdm_jt r = Predicted_Signal(cdm_np, pj, cm, fxj) - (cdm_jt)mv[m];
         cdm_jt d[cdm_np];
+        const cdm_jt r = jac.eval(pj, cm, fxj, cdm_mc_row(s_mc, m), d) - (cdm_jt)mv[m];
+        cdm_lm_acc df[cdm_np];
 #pragma unroll
-        for (int i = 0; i < cdm_np; ++i) d[i] = (cdm_jt)0;
-        Partial_Derivatives(cdm_np, pj, cm, fxj, d);
-        float df[cdm_np];
-#pragma unroll
-        for (int i = 0; i < cdm_np; ++i) df[i] = (float)(d[i] * dtf[i]);
-        const float rf = (float)r;
+        for (int i = 0; i < cdm_np; ++i) df[i] = (cdm_lm_acc)(d[i] * dtf[i]);
+        const cdm_lm_acc rf = (cdm_lm_acc)r;
 #pragma unroll
         for (int i = 0; i < cdm_np; ++i) gl[i] += rf * df[i];
         int k = 0;
@@ -442,8 +562,8 @@ __global__ void __launch_bounds__(cdm_block) cdm_levmar_kernel(cdm_lm_args a, in
       for (int i = 0; i < cdm_np; ++i)
 #pragma unroll
         for (int j = 0; j <= i; ++j, ++k) {
-          float h = 2.f * cdm::Group<L>::sum(hs[k]);
-          if (a.c.fixed[i] || a.c.fixed[j]) h = (i == j) ? 1.f : 0.f;
+          cdm_lm_acc h = (cdm_lm_acc)2 * cdm::Group<L>::sum(hs[k]);
+          if (a.c.fixed[i] || a.c.fixed[j]) h = (i == j) ? (cdm_lm_acc)1 : (cdm_lm_acc)0;
           hs[k] = h;
         }
 #pragma unroll
@@ -456,9 +576,12 @@ __global__ void __launch_bounds__(cdm_block) cdm_levmar_kernel(cdm_lm_args a, in
       if (marquardt) diag[i] = ((1 + lambda) / (1 + olambda)) * diag[i];
       else diag[i] += lambda - olambda;
     }
-    // FP64 Cholesky of the damped matrix, then two triangular solves.
-    double fac[cdm_ntri];
-    bool spd = true;
+    // FP64 LDL^T factorisation of the damped matrix (square-root-free
+    // Cholesky).  Its pivots are those of Gaussian elimination without
+    // pivoting, so like the reference's Gauss-Jordan solve it gives a step
+    // for any matrix with non-zero pivots, and inf/NaN (a rejected step)
+    // for a zero pivot (Levenberg_Marquardt.cu:336-407).
+    double fac[cdm_ntri], dg[cdm_np];
     {
       int k = 0;
 #pragma unroll
@@ -467,42 +590,42 @@ __global__ void __launch_bounds__(cdm_block) cdm_levmar_kernel(cdm_lm_args a, in
         for (int j = 0; j <= i; ++j, ++k) fac[k] = (i == j) ? diag[i] : (double)hs[k];
 #pragma unroll
       for (int j = 0; j < cdm_np; ++j) {
-        double s = fac[j * (j + 1) / 2 + j];
+        double dj = fac[j * (j + 1) / 2 + j];
 #pragma unroll
-        for (int q = 0; q < j; ++q) s -= fac[j * (j + 1) / 2 + q] * fac[j * (j + 1) / 2 + q];
-        spd = spd && (s > 0.0);
-        const double ljj = sqrt(s);
-        fac[j * (j + 1) / 2 + j] = ljj;
-        const double inv = 1.0 / ljj;
+        for (int q = 0; q < j; ++q) dj -= fac[j * (j + 1) / 2 + q] * fac[j * (j + 1) / 2 + q] * dg[q];
+        dg[j] = dj;
+        const double inv = 1.0 / dj;
 #pragma unroll
         for (int i = j + 1; i < cdm_np; ++i) {
           double x = fac[i * (i + 1) / 2 + j];
 #pragma unroll
-          for (int q = 0; q < j; ++q) x -= fac[i * (i + 1) / 2 + q] * fac[j * (j + 1) / 2 + q];
+          for (int q = 0; q < j; ++q) x -= fac[i * (i + 1) / 2 + q] * fac[j * (j + 1) / 2 + q] * dg[q];
           fac[i * (i + 1) / 2 + j] = x * inv;
         }
       }
     }
     double step[cdm_np];
 #pragma unroll
-    for (int i = 0; i < cdm_np; ++i) {
+    for (int i = 0; i < cdm_np; ++i) {  // L y = g
       double x = grad[i];
 #pragma unroll
       for (int q = 0; q < i; ++q) x -= fac[i * (i + 1) / 2 + q] * step[q];
-      step[i] = x / fac[i * (i + 1) / 2 + i];
+      step[i] = x;
     }
 #pragma unroll
-    for (int i = cdm_np - 1; i >= 0; --i) {
+    for (int i = 0; i < cdm_np; ++i) step[i] /= dg[i];  // D z = y
+#pragma unroll
+    for (int i = cdm_np - 1; i >= 0; --i) {  // L^T x = z
       double x = step[i];
 #pragma unroll
       for (int q = i + 1; q < cdm_np; ++q) x -= fac[q * (q + 1) / 2 + i] * step[q];
-      step[i] = x / fac[i * (i + 1) / 2 + i];
+      step[i] = x;
     }
     double tn[cdm_np];
 #pragma unroll
-    for (int i = 0; i < cdm_np; ++i) tn[i] = spd ? t[i] - step[i] : NAN;
+    for (int i = 0; i < cdm_np; ++i) tn[i] = t[i] - step[i];
     cdm_transform(a.c, p, tn);
-    const double ncf = cdm_sse<L>(a.g, s_cfp, mv, p, fxd, lane);
+    const double ncf = cdm_sse<L>(a.g, s_cfp, s_mc, mv, p, fxd, lane);
     bool end = false;
     success = ncf < pcf;
     if (success) {
@@ -584,15 +707,16 @@ __device__ __forceinline__ double cdm_logI0(double x) {
 }
 
 template <int L, bool RICIAN>
-__device__ __forceinline__ double cdm_neg_loglik(const cdm_geom& g, const float* s_cfp, const float* mv,
-                                                 const double* p, double tau, cdm_tf* fx, int lane) {
+__device__ __forceinline__ double cdm_neg_loglik(const cdm_geom& g, const float* s_cfp, const CDM_F64* s_mc,
+                                                 const float* mv, const double* p, double tau, cdm_tf* fx,
+                                                 const CDM_F64* vc, int lane) {
   cdm_tf pf[cdm_np];
 #pragma unroll
   for (int i = 0; i < cdm_np; ++i) pf[i] = (cdm_tf)p[i];
   cdm_cfp_row<cdm_tf> row;
   double acc = 0.0;
   for (int m = lane; m < g.nmeas; m += L) {
-    const cdm_tf s = Predicted_Signal(cdm_np, pf, row.at(s_cfp, m, g.cfp_ts), fx);
+    const cdm_tf s = cdm_signal(pf, row.at(s_cfp, m, g.cfp_ts), fx, vc, cdm_mc_row(s_mc, m));
     if constexpr (RICIAN) {
       const double y = mv[m], sd = (double)s;
       acc += log(y) + (-0.5 * tau * (y * y + sd * sd) + cdm_logI0(tau * sd * y));
@@ -604,6 +728,14 @@ __device__ __forceinline__ double cdm_neg_loglik(const cdm_geom& g, const float*
   acc = cdm::Group<L>::sum(acc);
   if constexpr (RICIAN) return -g.nmeas * log(tau) - acc;
   else return (g.nmeas / 2.0) * log(acc / 2.0);
+}
+
+// Voxel cache of a double parameter vector, computed in the plug-in's precision.
+__device__ __forceinline__ void cdm_prepare_d(const double* p, cdm_tf* fx, CDM_F64* vc) {
+  cdm_tf pf[cdm_np];
+#pragma unroll
+  for (int i = 0; i < cdm_np; ++i) pf[i] = (cdm_tf)p[i];
+  cdm_prepare(pf, fx, vc);
 }
 
 template <class TP>
@@ -631,7 +763,7 @@ __device__ __forceinline__ double cdm_prior(const cdm_consts& c, int i, TP* p, i
 template <int L, bool RICIAN>
 __global__ void __launch_bounds__(cdm_block) cdm_mcmc_kernel(cdm_mcmc_args a) {
   extern __shared__ float s_cfp[];
-  cdm_stage_cfp(a.g, s_cfp);
+  const CDM_F64* s_mc = cdm_stage_cfp(a.g, s_cfp);
   const int v = blockIdx.x * (cdm_block / L) + threadIdx.x / L;
   const int lane = threadIdx.x % L;
   if (v >= a.g.nvox) return;
@@ -677,15 +809,18 @@ __global__ void __launch_bounds__(cdm_block) cdm_mcmc_kernel(cdm_mcmc_args a) {
       cdm_tf pf[cdm_np];
 #pragma unroll
       for (int i = 0; i < cdm_np; ++i) pf[i] = (cdm_tf)p[i];
+      CDM_F64 vc0[cdm_vc];
+      cdm_prepare(pf, fx, vc0);
       cdm_cfp_row<cdm_tf> row;
       double s1 = 0.0;
       for (int m = lane; m < a.g.nmeas; m += L)
-        s1 += (double)(Predicted_Signal(cdm_np, pf, row.at(s_cfp, m, a.g.cfp_ts), fx) - (cdm_tf)mv[m]);
+        s1 += (double)(cdm_signal(pf, row.at(s_cfp, m, a.g.cfp_ts), fx, vc0, cdm_mc_row(s_mc, m)) - (cdm_tf)mv[m]);
       const double mean = cdm::Group<L>::sum(s1) / a.g.nmeas;
       double s2 = 0.0;
       for (int m = lane; m < a.g.nmeas; m += L) {
-        const double e = mean - (double)(Predicted_Signal(cdm_np, pf, row.at(s_cfp, m, a.g.cfp_ts), fx) -
-                                         (cdm_tf)mv[m]);
+        const double e =
+            mean - (double)(cdm_signal(pf, row.at(s_cfp, m, a.g.cfp_ts), fx, vc0, cdm_mc_row(s_mc, m)) -
+                            (cdm_tf)mv[m]);
         s2 += e * e;
       }
       tau = 1.0 / (cdm::Group<L>::sum(s2) / a.g.nmeas);
@@ -694,7 +829,9 @@ __global__ void __launch_bounds__(cdm_block) cdm_mcmc_kernel(cdm_mcmc_args a) {
     double tp = 0.0;
 #pragma unroll
     for (int i = 0; i < cdm_np; ++i) tp += prior[i];
-    energy = tp + cdm_neg_loglik<L, RICIAN>(a.g, s_cfp, mv, p, tau, fx, lane);
+    CDM_F64 vc0[cdm_vc];
+    cdm_prepare_d(p, fx, vc0);
+    energy = tp + cdm_neg_loglik<L, RICIAN>(a.g, s_cfp, s_mc, mv, p, tau, fx, vc0, lane);
   } else {
 #pragma unroll
     for (int i = 0; i < cdm_np; ++i) {
@@ -711,6 +848,9 @@ __global__ void __launch_bounds__(cdm_block) cdm_mcmc_kernel(cdm_mcmc_args a) {
     tau_rej = st->tau_rej;
   }
 
+  // voxel cache of the current state (a deterministic function of p)
+  CDM_F64 vc_cur[cdm_vc];
+  cdm_prepare_d(p, fx, vc_cur);
   const int end = a.begin + a.count;
   for (int it = a.begin; it < end; ++it) {
     const uint32_t jump = a.jump_offset + (uint32_t)it;
@@ -722,7 +862,7 @@ __global__ void __launch_bounds__(cdm_block) cdm_mcmc_kernel(cdm_mcmc_args a) {
         double tp = 0.0;
 #pragma unroll
         for (int i = 0; i < cdm_np; ++i) tp += prior[i];
-        const double e_new = tp + cdm_neg_loglik<L, true>(a.g, s_cfp, mv, p, tau, fx, lane);
+        const double e_new = tp + cdm_neg_loglik<L, true>(a.g, s_cfp, s_mc, mv, p, tau, fx, vc_cur, lane);
         if (exp(energy - e_new) > d.uniform) {
           ++tau_acc;
           energy = e_new;
@@ -745,7 +885,11 @@ __global__ void __launch_bounds__(cdm_block) cdm_mcmc_kernel(cdm_mcmc_args a) {
       if ((bt == cdm_BMIN || bt == cdm_BMINMAX) && p[i] < a.c.bmin[i]) ok = false;
       if ((bt == cdm_BMAX || bt == cdm_BMINMAX) && p[i] > a.c.bmax[i]) ok = false;
       if (ok) {
-        const double like = cdm_neg_loglik<L, RICIAN>(a.g, s_cfp, mv, p, tau, fx, lane);
+        // the voxel cache changes only with the parameters it depends on
+        const bool dep = (cdm_vc_dep >> i) & 1u;
+        CDM_F64 vc_new[cdm_vc];
+        if (dep) cdm_prepare_d(p, fx, vc_new);
+        const double like = cdm_neg_loglik<L, RICIAN>(a.g, s_cfp, s_mc, mv, p, tau, fx, dep ? vc_new : vc_cur, lane);
         const double old_prior = prior[i];
         prior[i] = cdm_prior(a.c, i, p, a.g.nmeas, s_cfp, fx);
         double tp = 0.0;
@@ -755,6 +899,10 @@ __global__ void __launch_bounds__(cdm_block) cdm_mcmc_kernel(cdm_mcmc_args a) {
         if (exp(energy - e_new) > d.uniform) {
           ++nacc[i];
           energy = e_new;
+          if (dep) {
+#pragma unroll
+            for (int k = 0; k < cdm_vc; ++k) vc_cur[k] = vc_new[k];
+          }
         } else {
           ++nrej[i];
           p[i] = old;
@@ -840,7 +988,7 @@ __device__ __forceinline__ void cdm_mean_sample(const cdm_post_args& a, int v, d
 template <int L>
 __global__ void __launch_bounds__(cdm_block) cdm_predicted_kernel(cdm_post_args a) {
   extern __shared__ float s_cfp[];
-  cdm_stage_cfp(a.g, s_cfp);
+  const CDM_F64* s_mc = cdm_stage_cfp(a.g, s_cfp);
   const int v = blockIdx.x * (cdm_block / L) + threadIdx.x / L;
   const int lane = threadIdx.x % L;
   if (v >= a.g.nvox) return;
@@ -852,14 +1000,16 @@ __global__ void __launch_bounds__(cdm_block) cdm_predicted_kernel(cdm_post_args 
 #pragma unroll
   for (int i = 0; i < cdm_np; ++i) pf[i] = (cdm_tf)mp[i];
   cdm_cfp_row<cdm_tf> row;
+  CDM_F64 vc[cdm_vc];
+  cdm_prepare(pf, fx, vc);
   for (int m = lane; m < a.g.nmeas; m += L)
-    a.out[(size_t)v * a.g.nmeas + m] = (float)Predicted_Signal(cdm_np, pf, row.at(s_cfp, m, a.g.cfp_ts), fx);
+    a.out[(size_t)v * a.g.nmeas + m] = (float)cdm_signal(pf, row.at(s_cfp, m, a.g.cfp_ts), fx, vc, cdm_mc_row(s_mc, m));
 }
 
 template <int L, bool RICIAN>
 __global__ void __launch_bounds__(cdm_block) cdm_bic_aic_kernel(cdm_post_args a) {
   extern __shared__ float s_cfp[];
-  cdm_stage_cfp(a.g, s_cfp);
+  const CDM_F64* s_mc = cdm_stage_cfp(a.g, s_cfp);
   const int v = blockIdx.x * (cdm_block / L) + threadIdx.x / L;
   const int lane = threadIdx.x % L;
   if (v >= a.g.nvox) return;
@@ -879,7 +1029,9 @@ __global__ void __launch_bounds__(cdm_block) cdm_bic_aic_kernel(cdm_post_args a)
       tau = ts[0];
     }
   }
-  const double nll = cdm_neg_loglik<L, RICIAN>(a.g, s_cfp, mv, mp, tau, fx, lane);
+  CDM_F64 vc[cdm_vc];
+  cdm_prepare_d(mp, fx, vc);
+  const double nll = cdm_neg_loglik<L, RICIAN>(a.g, s_cfp, s_mc, mv, mp, tau, fx, vc, lane);
   // nll is -loglik (Rician) or (M/2) log(SSE/2) (Gaussian): loglik = -nll.
   const int np = cdm_np + (RICIAN ? 1 : 0);
   if (lane == 0) {

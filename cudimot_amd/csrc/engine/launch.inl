@@ -28,7 +28,9 @@ static cdm_geom op_geom(const cdm::Geom& g) {
   return k;
 }
 
-static size_t op_lds(const cdm::Geom& g) { return (size_t)(g.nmeas * g.cfp_ts + cdm_cfp_max) * sizeof(float); }
+static size_t op_lds(const cdm::Geom& g) {
+  return (size_t)cdm_mc_offset(g.nmeas, g.cfp_ts) * sizeof(float) + (size_t)g.nmeas * cdm_mc * sizeof(CDM_F64);
+}
 
 static dim3 op_grid(int nvox, int L) {
   const int per_block = cdm_block / L;

@@ -1,5 +1,10 @@
 """Algorithmic work model of the fitting methods (FLOPs by precision).
 
+Invariants are counted once where they are invariant: a plug-in's voxel
+cache (Voxel_Cache) once per parameter vector, the per-measurement cache
+(Meas_Cache) once per launch (not counted), and the cached per-measurement
+signal (Predicted_Signal_Cached) per evaluation.
+
 Per forward evaluation F_fwd and per Partial_Derivatives call F_jac are
 counted (not estimated) by running each plug-in on the CPU with an
 operation-counting scalar (lib/libcdm_opcount.so, csrc/tools/opcount.h):
@@ -37,18 +42,25 @@ def per_eval(model, params, cfp, fixp=None):
     cfp = np.ascontiguousarray(cfp, dtype=np.float64)
     n = params.shape[0]
     fixp = np.ascontiguousarray(fixp if fixp is not None else np.zeros((n, 1)), dtype=np.float64)
-    out = np.zeros(8)
+    out = np.zeros(14)
     D = ctypes.POINTER(ctypes.c_double)
     getattr(_load(), "cdm_opcount_" + model)(n, cfp.shape[0], params.ctypes.data_as(D), cfp.ctypes.data_as(D),
                                              fixp.ctypes.data_as(D), out.ctypes.data_as(D))
     return {"fwd32": out[0], "fwd64": out[1], "fwd_t32": out[2], "fwd_t64": out[3],
-            "jac32": out[4], "jac64": out[5], "jac_t32": out[6], "jac_t64": out[7]}
+            "jac32": out[4], "jac64": out[5], "jac_t32": out[6], "jac_t64": out[7],
+            "vc32": out[8], "vc64": out[9], "vc_dep": int(out[10]), "cached": bool(out[11]),
+            "meas32": out[12], "meas64": out[13]}
 
 
 def mcmc_flops(ev, nparams, nmeas, jumps):
-    """(FP32, FP64) flops of one voxel's MCMC over `jumps` jumps."""
+    """(FP32, FP64) flops of one voxel's MCMC over `jumps` jumps: every
+    proposal evaluates the signal at all measurements, and recomputes the
+    voxel cache when the proposed parameter is one it depends on."""
     n = jumps * nparams * nmeas
-    return n * (ev["fwd32"] + 3), n * (ev["fwd64"] + 1)
+    ndep = bin(ev["vc_dep"]).count("1") if ev["cached"] else 0
+    f32 = n * (ev["meas32"] + 3) + jumps * ndep * ev["vc32"]
+    f64 = n * (ev["meas64"] + 1) + jumps * ndep * ev["vc64"]
+    return f32, f64
 
 
 def lm_flops(ev, nparams, nmeas, iters, jac_double=False):

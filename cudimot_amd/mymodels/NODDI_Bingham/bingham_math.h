@@ -102,10 +102,11 @@ __device__ inline void bingham_fan_setup(CDM_F64 fintra, CDM_F64 kappa, CDM_F64 
   f.dperp2 = CDM_NODDI_DPAR + 2 * dperp - f.dpar - f.dperp1;
 }
 
-// Signal of one fan (1-fintra) E + fintra I for gradient g and b-value b.
-// Frame: fibre n(th, ph), fanning direction from psi, and their cross
-// product (NODDI_Bingham/modelfunctions.h:116-138).
-MACRO T bingham_fan_signal(T fintra, T kappa, T beta, T th, T ph, T psi, const bingham_fan& f, T* CFP) {
+// Hindered (e) and intra-axonal (i) signals of one fan for gradient g and
+// b-value b.  Frame: fibre n(th, ph), fanning direction from psi, and their
+// cross product (NODDI_Bingham/modelfunctions.h:116-138).
+MACRO void bingham_fan_signal_parts(T kappa, T beta, T th, T ph, T psi, const bingham_fan& f, T* CFP, T& extra,
+                                    T& intra) {
   T st, ct, sp, cp, ss, cs;
   tsincos(th, &st, &ct);
   tsincos(ph, &sp, &cp);
@@ -122,13 +123,18 @@ MACRO T bingham_fan_signal(T fintra, T kappa, T beta, T th, T ph, T psi, const b
   const T az = tatan2(gy, gx);
   const T caz = cos_gpu(az);
   const T caz2 = caz * caz, saz2 = (T)1 - caz2;
-  const T extra = exp_gpu(-b * ((T)f.dpar * c2 + s2 * ((T)f.dperp1 * saz2 + (T)f.dperp2 * caz2)));
+  extra = exp_gpu(-b * ((T)f.dpar * c2 + s2 * ((T)f.dperp1 * saz2 + (T)f.dperp2 * caz2)));
   // intra-axonal compartment: 1F1 of diag(0, beta, kappa) - b Dpar g_local g_local^T
   const CDM_F64 pc = -(CDM_F64)b * CDM_NODDI_DPAR;
   const CDM_F64 dx = (CDM_F64)gx, dy = (CDM_F64)gy, dn = (CDM_F64)gn;
   CDM_F64 e[3];
   bingham_eig3(dx * dx * pc, dx * dy * pc, dx * dn * pc, (CDM_F64)beta + dy * dy * pc, dy * dn * pc,
                (CDM_F64)kappa + dn * dn * pc, e);
-  const T intra = (T)(bingham_hyp(e[0], e[1], e[2]) / f.nc);
-  return ((T)1 - fintra) * extra + fintra * intra;
+  intra = (T)(bingham_hyp(e[0], e[1], e[2]) / f.nc);
+}
+
+MACRO T bingham_fan_signal(T fintra, T kappa, T beta, T th, T ph, T psi, const bingham_fan& f, T* CFP) {
+  T e, i;
+  bingham_fan_signal_parts(kappa, beta, th, ph, psi, f, CFP, e, i);
+  return ((T)1 - fintra) * e + fintra * i;
 }

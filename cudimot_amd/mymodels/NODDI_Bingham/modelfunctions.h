@@ -5,29 +5,66 @@
 // Follows NODDI_Bingham/modelfunctions.h:37-246: analytic derivatives in
 // fiso and fintra (the latter ignoring the tortuosity dependence of E, as
 // the reference), central differences for the rest.
+//
+// Engine cache hooks: the normaliser 1F1(kappa, beta, 0) and the hindered
+// diffusivities (five 1F1 evaluations) depend on fintra, kappa and beta
+// only, so they are computed once per parameter vector (Voxel_Cache) instead
+// of once per measurement.
 #include "bingham_math.h"
 
-MACRO T Predicted_Signal(int npar, T* P, T* CFP, T* FixP) {
+#define CDM_VOXEL_CACHE 4
+#define CDM_VOXEL_CACHE_DEPENDS 0x0Eu  // fintra, kappa, beta
+#define CDM_NUMERICAL_MASK 0x7Cu       // kappa, beta, th, ph, psi
+
+MACRO void Voxel_Cache(int npar, T* P, T* FixP, CDM_F64* vc) {
   bingham_fan f;
   bingham_fan_setup((CDM_F64)P[1], (CDM_F64)P[2], (CDM_F64)P[3], f);
-  const T fan = bingham_fan_signal(P[1], P[2], P[3], P[4], P[5], P[6], f, CFP);
+  vc[0] = f.nc;
+  vc[1] = f.dpar;
+  vc[2] = f.dperp1;
+  vc[3] = f.dperp2;
+}
+
+MACRO void bingham1_parts(T* P, T* CFP, const CDM_F64* vc, T& fan, T& e, T& i) {
+  bingham_fan f;
+  f.nc = vc[0];
+  f.dpar = vc[1];
+  f.dperp1 = vc[2];
+  f.dperp2 = vc[3];
+  bingham_fan_signal_parts(P[2], P[3], P[4], P[5], P[6], f, CFP, e, i);
+  fan = ((T)1 - P[1]) * e + P[1] * i;
+}
+
+MACRO T Predicted_Signal_Cached(int npar, T* P, T* CFP, T* FixP, const CDM_F64* vc, const CDM_F64* mc) {
+  T fan, e, i;
+  bingham1_parts(P, CFP, vc, fan, e, i);
   const T iso = exp_gpu(-CFP[3] * (T)CDM_NODDI_DISO);
   return FixP[0] * (((T)1 - P[0]) * fan + P[0] * iso);
+}
+
+MACRO T Predicted_Signal(int npar, T* P, T* CFP, T* FixP) {
+  CDM_F64 vc[4];
+  Voxel_Cache(npar, P, FixP, vc);
+  return Predicted_Signal_Cached(npar, P, CFP, FixP, vc, (const CDM_F64*)0);
 }
 
 // kappa must exceed beta by 0.05 (NODDI_Bingham/modelfunctions.h:169-178).
 MACRO bool ConstraintsMCMC(int npar, T* P) { return !((P[2] - (T)0.05) < P[3]); }
 
-MACRO void Partial_Derivatives(int npar, T* P, T* CFP, T* FixP, T* derivatives) {
-  bingham_fan f;
-  bingham_fan_setup((CDM_F64)P[1], (CDM_F64)P[2], (CDM_F64)P[3], f);
-  const T fan = bingham_fan_signal(P[1], P[2], P[3], P[4], P[5], P[6], f, CFP);
-  // compartments for the fintra derivative: fan = (1-fintra) E + fintra I
-  const T e = bingham_fan_signal((T)0, P[2], P[3], P[4], P[5], P[6], f, CFP);
-  const T i = bingham_fan_signal((T)1, P[2], P[3], P[4], P[5], P[6], f, CFP);
+// Analytic entries (fiso, fintra); the engine forms the others.
+MACRO void Partial_Derivatives_Cached(int npar, T* P, T* CFP, T* FixP, const CDM_F64* vc, const CDM_F64* mc,
+                                      T* derivatives) {
+  T fan, e, i;
+  bingham1_parts(P, CFP, vc, fan, e, i);
   const T iso = exp_gpu(-CFP[3] * (T)CDM_NODDI_DISO);
   derivatives[0] = FixP[0] * (iso - fan);
   derivatives[1] = FixP[0] * ((T)1 - P[0]) * (i - e);
+}
+
+MACRO void Partial_Derivatives(int npar, T* P, T* CFP, T* FixP, T* derivatives) {
+  CDM_F64 vc[4];
+  Voxel_Cache(npar, P, FixP, vc);
+  Partial_Derivatives_Cached(npar, P, CFP, FixP, vc, (const CDM_F64*)0, derivatives);
   derivatives[2] = NUMERICAL(2);
   derivatives[3] = NUMERICAL(3);
   derivatives[4] = NUMERICAL(4);

@@ -164,7 +164,7 @@ def _run_levmar(torch_cuda, model, lanes, start, n=None):
     m = capi.Model(model)
     m.set_group(capi.LEVMAR, lanes)
     spec = m.spec()
-    n = n or {"Ball_2_Sticks": 256, "Ball_3_Sticks": 256, "NODDI_Watson": 256, "NODDI_Bingham": 96}.get(model, 1024)
+    n = n or {"Ball_2_Sticks": 256, "Ball_3_Sticks": 256, "NODDI_Watson": 256, "NODDI_Bingham": 192}.get(model, 1024)
     cfp, t, fx, noisy = H.volume(o, model, n, seed=31)
     p0 = _start(model, spec, noisy, cfp, t) if start == "init" else _near_truth(model, t, np.random.default_rng(4))
     noisy32 = noisy.astype(np.float32).astype(np.float64)
