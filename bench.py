@@ -34,11 +34,9 @@ PEAK_F32 = 157.3e12
 
 def make_volume(m, n, seed, dev, torch):
     from cudimot_amd import synth
-    cfp = synth.acquisition(9, [(700, 30), (2000, 60)], seed=12)
-    t = synth.truth_noddi_bingham(n, seed)
+    cfp = synth.model_acquisition("NODDI_Bingham")
     # inside the model's bounds and constraint (beta <= 32, kappa - beta > 0.05)
-    t[:, 2] = np.clip(t[:, 2], 0.2, 63.0)
-    t[:, 3] = np.clip(np.minimum(t[:, 3], t[:, 2] - 0.2), 0.02, 31.0)
+    t = synth.truth_for("NODDI_Bingham", n, seed)
     d_cfp = dev(cfp)
     d_fx = torch.full((n, 1), 1000.0, device="cuda")
     clean = torch.empty((n, cfp.shape[0]), device="cuda")
@@ -125,12 +123,17 @@ def main():
 
     cfp, truth, d_cfp, d_fx, meas, p0 = make_volume(m, args.voxels, args.seed, dev, torch)
     torch.cuda.synchronize()
+    print("bench: warm-up done, timing %d step(s) of %d voxels" % (args.steps, args.voxels), file=sys.stderr,
+          flush=True)
     mcmc_ms = []
     t0 = time.time()
-    for _ in range(args.steps):
+    for k in range(args.steps):
         ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
         step(meas, p0, d_cfp, d_fx, args.voxels, ev)
         mcmc_ms.append(ev)
+        if k + 1 < args.steps:
+            torch.cuda.synchronize()
+            print("bench: step %d done at %.1f s" % (k + 1, time.time() - t0), file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.time() - t0
     mcmc_s = np.mean([a.elapsed_time(b) for a, b in mcmc_ms]) / 1000.0
@@ -159,6 +162,7 @@ def main():
            "roofline": roofline}
     if args.cpu_sample > 0:
         ns = args.cpu_sample
+        print("bench: GPU done (%.1f voxels/s); CPU oracle on %d voxels" % (value, ns), file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args, cfp, meas[:ns].cpu().numpy().astype(np.float64),
                                            p0[:ns].cpu().numpy().astype(np.float64), ns)
     print(json.dumps(out), flush=True)

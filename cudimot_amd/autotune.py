@@ -1,0 +1,110 @@
+"""Autotuner of the launch shape: voxel-group size (lanes per voxel) per
+model, measurement count and method.
+
+For every method it times the kernels at each compiled group size (1, 4,
+16, 64 lanes) on a synthetic volume large enough to fill the GPU, with a
+short representative workload (a few LM iterations, a few MCMC jumps), and
+keeps the fastest (GridSearch keeps the built-in choice: its cost depends
+on the user's grid).  Results go to a text table, one line per choice:
+
+    <model> <nmeas> <method> <lanes> <ms>
+
+which the C-ABI reads at model open (cdm_model_load_tuning) and this module
+applies.  Run: python -m cudimot_amd.autotune [--models A,B] [--voxels N]
+"""
+import argparse
+import os
+import time
+
+import numpy as np
+
+from . import capi, synth
+
+TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.txt")
+METHOD_NAMES = {capi.GRIDSEARCH: "gridsearch", capi.LEVMAR: "levmar", capi.MCMC: "mcmc",
+                capi.PREDICTED: "predicted", capi.BIC_AIC: "bic_aic"}
+LANES = (1, 4, 16, 64)
+
+
+def _volume(model, m, n, torch):
+    cfp = synth.model_acquisition(model)
+    t = synth.truth_for(model, n, seed=3)
+    fx = torch.full((n, 1), 1000.0, device="cuda")
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+    d_cfp = dev(cfp)
+    clean = torch.empty((n, cfp.shape[0]), device="cuda")
+    m.set_group(capi.PREDICTED, 16)
+    m.predicted_signal(dev(t.reshape(n, -1, 1)), d_cfp, fx, clean)
+    noise = 0.03 * clean.abs().max()
+    meas = torch.sqrt((clean + noise * torch.randn_like(clean)) ** 2 + (noise * torch.randn_like(clean)) ** 2)
+    return cfp, dev(t), d_cfp, fx, meas.contiguous()
+
+
+def tune_model(model, n=65536, reps=2):
+    import torch
+    m = capi.Model(model)
+    cfp, t, d_cfp, fx, meas = _volume(model, m, n, torch)
+    P = m.nparams
+    samples = torch.empty((n, P, 2), device="cuda")
+    out = torch.empty_like(meas)
+    bic = torch.empty(n, device="cuda")
+    aic = torch.empty(n, device="cuda")
+    runs = {
+        capi.LEVMAR: lambda: m.levmar(meas, t.clone(), d_cfp, fx, max_iters=5),
+        capi.MCMC: lambda: m.mcmc(meas, t.clone(), d_cfp, fx, samples, None, nburn=16, njumps=16, sampleevery=8),
+        capi.PREDICTED: lambda: m.predicted_signal(samples, d_cfp, fx, out),
+        capi.BIC_AIC: lambda: m.bic_aic(meas, samples, d_cfp, fx, bic, aic),
+    }
+    res = {}
+    for method, fn in runs.items():
+        best = None
+        for L in LANES:
+            m.set_group(method, L)
+            fn()
+            torch.cuda.synchronize()
+            t0 = time.time()
+            for _ in range(reps):
+                fn()
+            torch.cuda.synchronize()
+            ms = (time.time() - t0) * 1000 / reps
+            if best is None or ms < best[1]:
+                best = (L, ms)
+        res[method] = best
+    return cfp.shape[0], res
+
+
+def write_table(rows, path=TABLE):
+    keep = []
+    if os.path.exists(path):
+        new_keys = {(r[0], r[1], r[2]) for r in rows}
+        for line in open(path):
+            f = line.split()
+            if len(f) >= 4 and not line.startswith("#") and (f[0], int(f[1]), f[2]) not in new_keys:
+                keep.append(line.rstrip("\n"))
+    with open(path, "w") as fh:
+        fh.write("# model nmeas method lanes ms  (written by python -m cudimot_amd.autotune)\n")
+        for line in keep:
+            fh.write(line + "\n")
+        for model, nmeas, method, lanes, ms in rows:
+            fh.write("%s %d %s %d %.3f\n" % (model, nmeas, method, lanes, ms))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--models", default="")
+    ap.add_argument("--voxels", type=int, default=65536)
+    ap.add_argument("--out", default=TABLE, help="table to write (default: the library's own)")
+    a = ap.parse_args(argv)
+    models = [x for x in a.models.split(",") if x] or capi.list_models()
+    rows = []
+    for model in models:
+        nmeas, res = tune_model(model, a.voxels)
+        for method, (L, ms) in res.items():
+            rows.append((model, nmeas, METHOD_NAMES[method], L, ms))
+        print(model, nmeas, {METHOD_NAMES[k]: v for k, v in res.items()}, flush=True)
+    write_table(rows, a.out)
+    print("wrote", a.out)
+
+
+if __name__ == "__main__":
+    main()

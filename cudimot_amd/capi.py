@@ -24,6 +24,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise CdmError("engine library not built: %s (run python -m cudimot_amd.build)" % LIB_PATH)
+        # PyTorch-ROCm wheels carry their own HIP/HSA runtime next to the one
+        # this library links (/opt/rocm).  Both share the process's GPU
+        # address space; the device is found by both only when torch's
+        # runtime is loaded first, so load it before this library.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         vp, i, u64, i64, cp = ctypes.c_void_p, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_longlong, ctypes.c_char_p
         ip, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
@@ -40,6 +48,7 @@ def lib():
             "cdm_model_grid": (i, [vp, ip, dp]),
             "cdm_model_set_group": (i, [vp, i, i]),
             "cdm_model_get_group": (i, [vp, i, i]),
+            "cdm_model_load_tuning": (i, [vp, cp]),
             "cdm_model_set_mcmc_blocks": (i, [vp, i, i]),
             "cdm_gridsearch": (i, [vp, i, i, vp, vp, vp, vp, vp, vp]),
             "cdm_levmar": (i, [vp, i, i, vp, vp, vp, vp, i, i, vp, vp, vp]),
@@ -141,6 +150,13 @@ class Model:
 
     def get_group(self, method, nmeas):
         r = lib().cdm_model_get_group(self.h, method, nmeas)
+        if r < 0:
+            raise CdmError(lib().cdm_last_error().decode())
+        return r
+
+    def load_tuning(self, path):
+        """Tuning table (lines '<model> <nmeas> <method> <lanes>'); None clears it."""
+        r = lib().cdm_model_load_tuning(self.h, path.encode() if path else None)
         if r < 0:
             raise CdmError(lib().cdm_last_error().decode())
         return r

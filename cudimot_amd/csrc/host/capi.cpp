@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -19,6 +21,11 @@ struct cdm_model {
   const cdm::ModelOps* ops = nullptr;
   cdm::ModelSpec spec;
   int group[5] = {0, 0, 0, 0, 0};
+  // autotuned choices for this model: (measurement count, method, lanes)
+  struct Tuned {
+    int nmeas, method, lanes;
+  };
+  std::vector<Tuned> tuned;
   int mcmc_block = 0;
   int mcmc_graph = 0;
   // device grid (valid combinations only)
@@ -80,18 +87,70 @@ int tsize(int n, const int* sizes) {
 
 bool valid_group(int L) { return L == 1 || L == 4 || L == 16 || L == 64; }
 
-// Built-in voxel-group choice: more lanes per voxel when a voxel has many
-// measurements, so each lane keeps a handful of them.
+// Built-in voxel-group choice when no tuned entry exists: more lanes per
+// voxel when a voxel has many measurements, so each lane keeps a handful of
+// them; LM keeps its normal matrix per voxel, so it favours wide groups.
 int default_group(int method, int nmeas) {
-  (void)method;
+  if (method == cdm::kLevMar) return nmeas >= 48 ? 64 : 16;
   if (nmeas >= 192) return 16;
-  if (nmeas >= 48) return 4;
   return 4;
 }
 
+// Explicit choice, else the tuned entry of the nearest measurement count
+// (log ratio), else the built-in rule.
 int group_for(const cdm_model* m, int method, int nmeas) {
-  const int g = m->group[method];
-  return g ? g : default_group(method, nmeas);
+  if (m->group[method]) return m->group[method];
+  int best = 0;
+  double dist = 0;
+  for (const auto& t : m->tuned) {
+    if (t.method != method) continue;
+    const double d = std::fabs(std::log((double)t.nmeas / (double)std::max(nmeas, 1)));
+    if (!best || d < dist) {
+      best = t.lanes;
+      dist = d;
+    }
+  }
+  return best ? best : default_group(method, nmeas);
+}
+
+const char* const kMethodNames[5] = {"gridsearch", "levmar", "mcmc", "predicted", "bic_aic"};
+
+// Tuning table lines: "<model> <nmeas> <method> <lanes> [ms]"; '#' starts a
+// comment.  Returns the number of entries kept for this model, -1 if the
+// file cannot be read.
+int read_tuning(cdm_model* m, const std::string& path, std::string* err) {
+  std::ifstream in(path);
+  if (!in) {
+    *err = "cannot read tuning table " + path;
+    return -1;
+  }
+  std::vector<cdm_model::Tuned> got;
+  std::string line;
+  int lineno = 0;
+  while (std::getline(in, line)) {
+    ++lineno;
+    const size_t h = line.find('#');
+    if (h != std::string::npos) line.resize(h);
+    std::istringstream ss(line);
+    std::string model, method;
+    int nmeas = 0, lanes = 0;
+    if (!(ss >> model)) continue;
+    if (!(ss >> nmeas >> method >> lanes) || nmeas <= 0 || !valid_group(lanes)) {
+      *err = path + ":" + std::to_string(lineno) + ": expected '<model> <nmeas> <method> <lanes>' with lanes 1/4/16/64";
+      return -1;
+    }
+    if (model != m->ops->name) continue;
+    int k = -1;
+    for (int i = 0; i < 5; ++i)
+      if (method == kMethodNames[i]) k = i;
+    if (k < 0) {
+      *err = path + ":" + std::to_string(lineno) + ": unknown method '" + method + "'";
+      return -1;
+    }
+    got.push_back({nmeas, k, lanes});
+  }
+  m->tuned = got;
+  return (int)got.size();
 }
 
 cdm::Geom make_geom(const cdm_model* m, int nvox, int nmeas, const float* meas, const float* cfp, const float* fixp) {
@@ -220,6 +279,13 @@ int cdm_model_open(const char* name, const char* priors_file, cdm_model** out) {
     delete m;
     return fail(err);
   }
+  // the library's own tuning table, when one was written next to it
+  std::string terr;
+  const std::string tpath = lib_dir() + "/../tuning.txt";
+  if (std::ifstream(tpath).good() && read_tuning(m, tpath, &terr) < 0) {
+    delete m;
+    return fail(terr);
+  }
   *out = m;
   return 0;
 }
@@ -287,6 +353,17 @@ int cdm_model_set_group(cdm_model* m, int method, int lanes) {
   if (lanes != 0 && !valid_group(lanes)) return fail("voxel-group size must be 1, 4, 16 or 64 lanes");
   m->group[method] = lanes;
   return 0;
+}
+
+int cdm_model_load_tuning(cdm_model* m, const char* path) {
+  if (!m) return fail("null model handle");
+  if (!path) {
+    m->tuned.clear();
+    return 0;
+  }
+  std::string err;
+  const int n = read_tuning(m, path, &err);
+  return n < 0 ? fail(err) : n;
 }
 
 int cdm_model_get_group(const cdm_model* m, int method, int nmeas) {

@@ -108,3 +108,58 @@ def truth_dti(n, seed, s0=1000.0):
 def truth_toymodel(n, seed):
     rng = np.random.default_rng(seed)
     return np.stack([rng.uniform(0.5, 2.0, n), rng.uniform(0.2, 2.0, n)], 1)
+
+
+def truth_noddi_bingham_2fans(n, seed):
+    """Two Bingham fans: fiso U[0,0.3], fFan2 U[0.1,0.5], per fan the
+    Bingham-NODDI draws above."""
+    t1 = truth_noddi_bingham(n, seed)
+    t2 = truth_noddi_bingham(n, seed + 1)
+    ff = np.random.default_rng(seed + 2).uniform(0.1, 0.5, n)
+    return np.concatenate([t1[:, :1], ff[:, None], t1[:, 1:], t2[:, 1:]], 1)
+
+
+ACQUISITIONS = {
+    # the measured workloads: (b=0 count, [(b, directions), ...], seed)
+    "M64": (4, [(1000, 30), (2000, 30)], 11),
+    "M99": (9, [(700, 30), (2000, 60)], 12),
+    "M280": (10, [(1000, 90), (2000, 90), (3000, 90)], 13),
+}
+
+MODEL_ACQUISITION = {"Ball_1_Stick": "M64", "Ball_2_Sticks": "M280", "Ball_3_Sticks": "M280",
+                     "DiffusionTensor": "M64", "NODDI_Watson": "M99", "NODDI_Bingham": "M99",
+                     "NODDI_Bingham_2Fans": "M99", "toymodel": None}
+
+
+def model_acquisition(model):
+    key = MODEL_ACQUISITION.get(model)
+    if key is None:
+        return np.linspace(0.0, 3.0, 20).reshape(-1, 1)
+    n0, shells, seed = ACQUISITIONS[key]
+    return acquisition(n0, shells, seed)
+
+
+def clip_to_bounds(model, t):
+    """Ground truths moved inside the shipped plug-ins' bounds and MCMC
+    constraints (Bingham: beta <= 32 and beta < kappa - 0.1)."""
+    t = t.copy()
+    if model == "NODDI_Bingham":
+        t[:, 2] = np.clip(t[:, 2], 0.2, 63.0)
+        t[:, 3] = np.clip(np.minimum(t[:, 3], t[:, 2] - 0.2), 0.02, 31.0)
+    elif model == "NODDI_Bingham_2Fans":
+        for k, b in ((3, 4), (9, 10)):
+            t[:, k] = np.clip(t[:, k], 0.2, 63.0)
+            t[:, b] = np.clip(np.minimum(t[:, b], t[:, k] - 0.6), 0.02, 31.0)
+    elif model == "NODDI_Watson":
+        t[:, 2] = np.clip(t[:, 2], 0.01, 63.0)
+    return t
+
+
+def truth_for(model, n, seed):
+    gen = {"Ball_1_Stick": truth_ball_1_stick,
+           "Ball_2_Sticks": lambda n, s: truth_ball_k_sticks(n, 2, s),
+           "Ball_3_Sticks": lambda n, s: truth_ball_k_sticks(n, 3, s),
+           "DiffusionTensor": truth_dti, "NODDI_Watson": truth_noddi_watson,
+           "NODDI_Bingham": truth_noddi_bingham, "NODDI_Bingham_2Fans": truth_noddi_bingham_2fans,
+           "toymodel": truth_toymodel}[model]
+    return clip_to_bounds(model, gen(n, seed))

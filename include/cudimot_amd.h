@@ -70,6 +70,15 @@ int cdm_model_grid(const cdm_model* m, int* grid_params, double* grid);
 int cdm_model_set_group(cdm_model* m, int method, int lanes);
 int cdm_model_get_group(const cdm_model* m, int method, int nmeas);
 
+/* Autotuned launch table (no counterpart in the reference, whose launch
+ * shape is fixed: one warp per voxel, 64 voxels per block, MCMC.cu:780-784).
+ * Lines "<model> <nmeas> <method> <lanes> [ms]", method one of gridsearch,
+ * levmar, mcmc, predicted, bic_aic; entries of other models are skipped.
+ * A method without an explicit set_group uses the entry of the nearest
+ * measurement count.  cdm_model_open loads <libdir>/../tuning.txt when it
+ * exists.  path NULL clears the table.  Returns the entries kept. */
+int cdm_model_load_tuning(cdm_model* m, const char* path);
+
 /* MCMC jump blocks: split each MCMC phase into launches of at most
  * `jumps_per_launch` jumps (0: one launch per phase); use_graph != 0
  * records the launches of a phase into a HIP graph and replays it. */

@@ -76,3 +76,31 @@ def test_rician_clean_matches_reference_rule():
     np.testing.assert_array_equal(x[0], [5, 3, 3, 3, 4])
     np.testing.assert_array_equal(x[1], [0, -2, 0, -1, 0])
     np.testing.assert_array_equal(x[2], [1, 2, 3, 4, 5])
+
+
+def test_tuning_table_nearest_measurement_count(tmp_path):
+    """Tuned entries apply per method at the nearest measurement count;
+    explicit set_group wins; a malformed table is an error."""
+    t = tmp_path / "tuning.txt"
+    t.write_text("# model nmeas method lanes ms\n"
+                 "NODDI_Bingham 99 levmar 64 1.0\n"
+                 "NODDI_Bingham 99 mcmc 4 2.0\n"
+                 "NODDI_Bingham 280 mcmc 16 3.0\n"
+                 "Ball_1_Stick 64 mcmc 1 1.0\n")
+    m = capi.Model("NODDI_Bingham")
+    assert m.load_tuning(str(t)) == 3
+    assert m.get_group(capi.LEVMAR, 99) == 64
+    assert m.get_group(capi.MCMC, 99) == 4
+    assert m.get_group(capi.MCMC, 120) == 4
+    assert m.get_group(capi.MCMC, 250) == 16
+    m.set_group(capi.MCMC, 64)
+    assert m.get_group(capi.MCMC, 99) == 64
+    m.set_group(capi.MCMC, 0)
+    assert m.get_group(capi.MCMC, 99) == 4
+    t.write_text("NODDI_Bingham 99 levmar 3\n")
+    with pytest.raises(capi.CdmError, match="lanes"):
+        m.load_tuning(str(t))
+    t.write_text("NODDI_Bingham 99 newton 4\n")
+    with pytest.raises(capi.CdmError, match="unknown method"):
+        m.load_tuning(str(t))
+    assert m.load_tuning(None) == 0
