@@ -104,6 +104,8 @@ def build(models=None, extra_dirs=(), force=False, jobs=None):
     jobs_list.append((reg, os.path.join(BUILD, "registry.o"), host_deps))
     for f in ("capi.cpp", "spec.cpp"):
         jobs_list.append((os.path.join(CSRC, "host", f), os.path.join(BUILD, f[:-4] + ".o"), host_deps))
+    jobs_list.append((os.path.join(CSRC, "mlp", "mlp.hip"), os.path.join(BUILD, "mlp.o"),
+                      host_deps + [os.path.join(CSRC, "mlp"), os.path.join(CSRC, "engine", "philox.h")]))
     if force:
         for _, o, _ in jobs_list:
             if os.path.exists(o + ".sha"):

@@ -56,6 +56,16 @@ def lib():
             "cdm_predicted_signal": (i, [vp, i, i, i, vp, vp, vp, vp, vp]),
             "cdm_bic_aic": (i, [vp, i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp]),
             "cdm_rician_clean": (i, [i, i, vp]),
+            "cdm_mlp_create": (i, [i, i, i, vp, i, u64, ctypes.POINTER(vp)]),
+            "cdm_mlp_destroy": (None, [vp]),
+            "cdm_mlp_info": (i, [vp, ip, ip, ip, ip, ctypes.POINTER(i64)]),
+            "cdm_mlp_get_weights": (i, [vp, vp]),
+            "cdm_mlp_set_weights": (i, [vp, vp]),
+            "cdm_mlp_forward": (i, [vp, i, i, vp, vp, vp, vp]),
+            "cdm_mlp_init_params": (i, [vp, i, i, vp, vp, i, vp, vp]),
+            "cdm_mlp_loss_grad": (i, [vp, i, i, vp, vp, i, vp, vp, vp, vp]),
+            "cdm_mlp_train": (i, [vp, i, i, vp, vp, i, vp, ctypes.c_float, ctypes.c_float, i, i, ctypes.c_float,
+                                  u64, vp, vp]),
             "cdm_sync": (i, [vp]),
         }
         for name, (res, args) in sig.items():
@@ -204,3 +214,83 @@ def rician_clean(meas_np):
     import numpy as np
     assert meas_np.dtype == np.float32 and meas_np.flags.c_contiguous
     check(lib().cdm_rician_clean(meas_np.shape[0], meas_np.shape[1], meas_np.ctypes.data))
+
+
+HEAD_SIGMOID, HEAD_SIGMOID_LOG, HEAD_SIGMOID_RATIO, HEAD_AXIS = range(4)
+
+
+class HeadEntry(ctypes.Structure):
+    """cdm_head_entry: how network outputs map to one model parameter."""
+    _fields_ = [("kind", ctypes.c_int), ("param", ctypes.c_int), ("param2", ctypes.c_int), ("lo", ctypes.c_float),
+                ("hi", ctypes.c_float), ("weight", ctypes.c_float), ("vmin", ctypes.c_float),
+                ("vmax", ctypes.c_float)]
+
+
+def head_entry(kind, param, param2=-1, lo=0.0, hi=1.0, weight=1.0, vmin=-3.0e38, vmax=3.0e38):
+    return HeadEntry(kind, param, param2, lo, hi, weight, vmin, vmax)
+
+
+class Mlp:
+    """Learned initialiser network (cdm_mlp_*)."""
+
+    def __init__(self, n_in, head, width=256, depth=3, seed=1):
+        arr = (HeadEntry * len(head))(*head)
+        h = ctypes.c_void_p()
+        check(lib().cdm_mlp_create(int(n_in), int(width), int(depth), ctypes.cast(arr, ctypes.c_void_p), len(head),
+                                   int(seed), ctypes.byref(h)))
+        self.h = h
+        self.head = list(head)
+        a, b, c, d = (ctypes.c_int() for _ in range(4))
+        nw = ctypes.c_longlong()
+        check(lib().cdm_mlp_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d),
+                                 ctypes.byref(nw)))
+        self.n_in, self.width, self.depth, self.n_out, self.nweights = a.value, b.value, c.value, d.value, nw.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.cdm_mlp_destroy(self.h)
+            self.h = None
+
+    def layer_shapes(self):
+        dims = [self.n_in] + [self.width] * self.depth + [self.n_out]
+        return [(dims[i + 1], dims[i]) for i in range(len(dims) - 1)]
+
+    def get_weights(self):
+        import numpy as np
+        w = np.empty(self.nweights, dtype=np.float32)
+        check(lib().cdm_mlp_get_weights(self.h, w.ctypes.data))
+        return w
+
+    def set_weights(self, w):
+        import numpy as np
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.nweights
+        check(lib().cdm_mlp_set_weights(self.h, w.ctypes.data))
+
+    def forward(self, meas, scale, out, stream=None):
+        n, nmeas = meas.shape
+        check(lib().cdm_mlp_forward(self.h, n, nmeas, ptr(meas), ptr(scale), ptr(out), _stream(stream)))
+
+    def init_params(self, meas, scale, params, stream=None):
+        n, nmeas = meas.shape
+        check(lib().cdm_mlp_init_params(self.h, n, nmeas, ptr(meas), ptr(scale), params.shape[1], ptr(params),
+                                        _stream(stream)))
+
+    def loss_grad(self, meas, scale, params_true, stream=None):
+        import numpy as np
+        n, nmeas = meas.shape
+        loss = np.zeros(1, dtype=np.float32)
+        grad = np.empty(self.nweights, dtype=np.float32)
+        check(lib().cdm_mlp_loss_grad(self.h, n, nmeas, ptr(meas), ptr(scale), params_true.shape[1],
+                                      ptr(params_true), loss.ctypes.data, grad.ctypes.data, _stream(stream)))
+        return float(loss[0]), grad
+
+    def train(self, clean, scale, params_true, snr=(10.0, 50.0), epochs=10, batch=4096, lr=1e-3, seed=1,
+              stream=None):
+        import numpy as np
+        n, nmeas = clean.shape
+        losses = np.zeros(max(epochs, 1), dtype=np.float32)
+        check(lib().cdm_mlp_train(self.h, n, nmeas, ptr(clean), ptr(scale), params_true.shape[1], ptr(params_true),
+                                  float(snr[0]), float(snr[1]), int(epochs), int(batch), float(lr), int(seed),
+                                  losses.ctypes.data, _stream(stream)))
+        return losses[:epochs]

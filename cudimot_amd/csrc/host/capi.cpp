@@ -245,6 +245,11 @@ int ensure_state(cdm_model* m, size_t bytes) {
 
 }  // namespace
 
+int cdm::set_error(const std::string& msg, int status) {
+  g_err = msg;
+  return status;
+}
+
 extern "C" {
 
 int cdm_version(void) { return 100; }

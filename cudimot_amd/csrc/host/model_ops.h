@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
+#include <string>
+
 namespace cdm {
 
 constexpr int kMaxParams = 32;
@@ -66,5 +68,10 @@ constexpr int kGroupSizes[] = {1, 4, 16, 64};
 const ModelOps* find_model(const char* name);
 int model_count();
 const ModelOps* model_at(int i);
+
+// Error reporting shared by every C-ABI source file: stores the message for
+// cdm_last_error() on this thread and returns the status (-1 argument error,
+// -2 HIP error).
+int set_error(const std::string& msg, int status = -1);
 
 }  // namespace cdm

@@ -1,0 +1,834 @@
+// Learned initialiser: a fully connected network n_in -> width^depth -> n_out
+// with GELU activations, evaluated and trained on the GPU with BF16 MFMA
+// products and FP32 accumulation.  It maps a voxel's normalised signal
+// (measurements divided by a per-voxel scale, e.g. S0) to a starting point
+// for Levenberg-Marquardt, standing in for GridSearch.  No counterpart in the
+// reference.
+//
+// Products (all "NT": both operands K-contiguous, C = A B^T):
+//   forward   H_l = gelu(H_{l-1} W_l^T + b_l)          A = H_{l-1} [B][K], B = W_l [N][K]
+//   backward  dH_{l-1} = dZ_l W_l                        A = dZ_l [B][N],   B = W_l^T [K][N]
+//             dW_l = dZ_l^T H_{l-1}                      A = dZ_l^T [N][B], B = H_{l-1}^T [K][B]
+// so each activation is kept in both layouts (the forward epilogue writes the
+// transposed copy), and each weight matrix in both layouts (the Adam step
+// writes them).  One wave computes a 32x32 tile with
+// v_mfma_f32_32x32x16_bf16 (operands read straight from global memory /
+// L2: every matrix here is at most 256 wide); a block of 4 waves covers 64x64.
+//
+// Output head: each model parameter the network predicts is one entry
+// (cdm_head_entry): a sigmoid onto [lo, hi] (linear or logarithmic), a
+// sigmoid ratio of an earlier parameter, or an axis (three outputs, the
+// direction of the output vector gives theta, phi).  Training minimises the
+// soft-target cross entropy of the sigmoid outputs plus w (1 - (v.n)^2/|v|^2)
+// for axes (sign-invariant: n and -n are the same fibre).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/cudimot_amd.h"
+#include "../engine/philox.h"
+#include "../host/model_ops.h"
+
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kMaxHead = 16;
+constexpr int kMaxLayers = 8;
+constexpr int kTile = 64;        // block tile (4 waves of 32x32)
+constexpr int kInferChunk = 65536;
+
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// ------------------------------------------------------------------ math
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * __expf(-0.5f * z * z);
+}
+__device__ __forceinline__ float sigmoid(float o) { return 1.0f / (1.0f + __expf(-o)); }
+
+// ------------------------------------------------------------------ GEMM
+enum { EPI_PLAIN = 0, EPI_FWD_GELU = 1, EPI_BWD_GELU = 2 };
+
+struct Epi {
+  const float* bias;  // [N] or null
+  float* out_f32;     // [M][ld_f32] (plain: C (+bias); forward: pre-activation z)
+  int ld_f32;
+  bf16* out_bf;       // [M][ld_bf]
+  int ld_bf;
+  bf16* out_bft;      // [N][ld_bft] transposed copy
+  int ld_bft;
+  const float* zin;   // [M][ld_zin] pre-activation (backward through gelu)
+  int ld_zin;
+};
+
+// C[M][N] = A[M][K] B[N][K]^T with M, N multiples of 32 and K of 16; lda, ldb
+// multiples of 8.  Grid (ceil(N/64), ceil(M/64)), block 256.
+// MFMA 32x32x16 operand layout (wave64): lane l supplies A row (l & 31) and
+// B row (l & 31), elements k0 + 8 (l >> 5) .. +7; accumulator element j of
+// lane l is C[8 (j >> 2) + 4 (l >> 5) + (j & 3)][l & 31].
+template <int EPI>
+__global__ void __launch_bounds__(256) gemm_nt(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B,
+                                               int ldb, int M, int N, int K, Epi e) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m0 = blockIdx.y * kTile + (wave >> 1) * 32;
+  const int n0 = blockIdx.x * kTile + (wave & 1) * 32;
+  if (m0 >= M || n0 >= N) return;
+  const int r = lane & 31, half = lane >> 5;
+  const bf16* pa = A + (size_t)(m0 + r) * lda + 8 * half;
+  const bf16* pb = B + (size_t)(n0 + r) * ldb + 8 * half;
+  f32x16 acc = {};
+#pragma unroll 4
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    const bf16x8 a = *(const bf16x8*)(pa + k0);
+    const bf16x8 b = *(const bf16x8*)(pb + k0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+  }
+  const int col = n0 + r;
+  float bias = (EPI != EPI_BWD_GELU && e.bias) ? e.bias[col] : 0.0f;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int row0 = m0 + 8 * g + 4 * half;
+    bf16x4 tv;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = row0 + i;
+      float v = acc[4 * g + i];
+      if (EPI == EPI_PLAIN) {
+        e.out_f32[(size_t)row * e.ld_f32 + col] = v + bias;
+      } else if (EPI == EPI_FWD_GELU) {
+        const float z = v + bias;
+        if (e.out_f32) e.out_f32[(size_t)row * e.ld_f32 + col] = z;
+        v = gelu(z);
+      } else {
+        v *= gelu_grad(e.zin[(size_t)row * e.ld_zin + col]);
+      }
+      if (EPI != EPI_PLAIN) {
+        const bf16 h = (bf16)v;
+        if (e.out_bf) e.out_bf[(size_t)row * e.ld_bf + col] = h;
+        tv[i] = h;
+      }
+    }
+    if (EPI != EPI_PLAIN && e.out_bft) *(bf16x4*)(e.out_bft + (size_t)col * e.ld_bft + row0) = tv;
+  }
+}
+
+// ------------------------------------------------------------------ inputs
+// x[v][k] = meas[v][k] / scale[v] for k < nmeas, 0 for the padding columns;
+// rows past nrows are zero.  Optional transposed copy xt[k][v].
+__global__ void inputs_kernel(int nrows, int rows_pad, int nmeas, int kpad, const float* __restrict__ meas,
+                              const float* __restrict__ scale, bf16* x, bf16* xt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows_pad * kpad) return;
+  const int v = i / kpad, k = i - v * kpad;
+  float val = 0.0f;
+  if (v < nrows && k < nmeas) val = meas[(size_t)v * nmeas + k] / (scale ? scale[v] : 1.0f);
+  const bf16 h = (bf16)val;
+  x[(size_t)v * kpad + k] = h;
+  if (xt) xt[(size_t)k * rows_pad + v] = h;
+}
+
+// Training inputs with fresh Rician noise: |S + s n1 + i s n2| / scale with
+// s = scale / SNR, SNR uniform in [snr_lo, snr_hi] per sample.  Draws are
+// Philox blocks addressed by (sample, measurement, epoch) under the seed.
+__global__ void noisy_inputs_kernel(int rows, int nmeas, int kpad, long long first, const float* __restrict__ clean,
+                                    const float* __restrict__ scale, float snr_lo, float snr_hi, uint64_t seed,
+                                    uint32_t epoch, bf16* x, bf16* xt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * kpad) return;
+  const int v = i / kpad, k = i - v * kpad;
+  float val = 0.0f;
+  if (k < nmeas) {
+    const uint64_t s = (uint64_t)(first + v);
+    const float sc = scale ? scale[s] : 1.0f;
+    cdm::U32x4 c;
+    c.v[0] = (uint32_t)s;
+    c.v[1] = (uint32_t)(s >> 32);
+    c.v[2] = 0xFFFFFFFFu;
+    c.v[3] = epoch;
+    const cdm::U32x4 ws = cdm::philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const float snr = snr_lo + (snr_hi - snr_lo) * (float)cdm::u01_open(ws.v[0]);
+    c.v[2] = (uint32_t)k;
+    const cdm::U32x4 w = cdm::philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const float u0 = (float)cdm::u01_open(w.v[0]), u1 = (float)cdm::u01_open(w.v[1]);
+    const float rad = sqrtf(-2.0f * logf(u0));
+    float sn, cs;
+    sincosf(6.2831853071795865f * u1, &sn, &cs);
+    const float sigma = sc / snr;
+    const float re = clean[(size_t)s * nmeas + k] + sigma * rad * cs, im = sigma * rad * sn;
+    val = sqrtf(re * re + im * im) / sc;
+  }
+  const bf16 h = (bf16)val;
+  x[(size_t)v * kpad + k] = h;
+  xt[(size_t)k * rows + v] = h;
+}
+
+// ------------------------------------------------------------------ head
+struct Head {
+  int n;
+  cdm_head_entry e[kMaxHead];
+  int out_index[kMaxHead];  // first network output of each entry
+};
+
+// Target encoding of the true parameters: per sample, n_out values.
+__global__ void encode_kernel(int n, int nparams, const float* __restrict__ params, Head h, int n_out,
+                              float* __restrict__ target) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const float* p = params + (size_t)v * nparams;
+  float* t = target + (size_t)v * n_out;
+  for (int j = 0; j < h.n; ++j) {
+    const cdm_head_entry& e = h.e[j];
+    const int o = h.out_index[j];
+    const float x = p[e.param];
+    if (e.kind == CDM_HEAD_AXIS) {
+      const float th = x, ph = p[e.param2];
+      t[o] = sinf(th) * cosf(ph);
+      t[o + 1] = sinf(th) * sinf(ph);
+      t[o + 2] = cosf(th);
+    } else {
+      float u;
+      if (e.kind == CDM_HEAD_SIGMOID) u = (x - e.lo) / (e.hi - e.lo);
+      else if (e.kind == CDM_HEAD_SIGMOID_LOG) u = (logf(x) - logf(e.lo)) / (logf(e.hi) - logf(e.lo));
+      else u = (x / p[e.param2] - e.lo) / (e.hi - e.lo);
+      t[o] = fminf(fmaxf(u, 0.0f), 1.0f);
+    }
+  }
+}
+
+// Loss and its gradient with respect to the raw outputs (times grad_scale,
+// the inverse batch size), written as BF16 in both layouts (padding columns
+// zero); loss_sum accumulates the loss times loss_scale.
+__global__ void loss_kernel(int rows, int n_out, int npad, const float* __restrict__ out,
+                            const float* __restrict__ target, Head h, float grad_scale, float loss_scale, bf16* dout,
+                            bf16* doutt, float* loss_sum) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.0f;
+  if (v < rows) {
+    const float* o = out + (size_t)v * npad;
+    const float* t = target + (size_t)v * n_out;
+    float g[32];
+    for (int k = 0; k < npad; ++k) g[k] = 0.0f;
+    for (int j = 0; j < h.n; ++j) {
+      const cdm_head_entry& e = h.e[j];
+      const int a = h.out_index[j];
+      if (e.kind == CDM_HEAD_AXIS) {
+        const float vx = o[a], vy = o[a + 1], vz = o[a + 2];
+        const float q = vx * t[a] + vy * t[a + 1] + vz * t[a + 2];
+        const float rr = vx * vx + vy * vy + vz * vz + 1e-6f;
+        l += e.weight * (1.0f - q * q / rr);
+        const float c1 = -2.0f * e.weight * q / rr, c2 = q / rr;
+        g[a] = c1 * (t[a] - c2 * vx);
+        g[a + 1] = c1 * (t[a + 1] - c2 * vy);
+        g[a + 2] = c1 * (t[a + 2] - c2 * vz);
+      } else {
+        const float s = sigmoid(o[a]), tt = t[a];
+        const float sc = fminf(fmaxf(s, 1e-7f), 1.0f - 1e-7f);
+        // soft-target cross entropy minus its minimum (0 at s = t)
+        const float tc = fminf(fmaxf(tt, 1e-7f), 1.0f - 1e-7f);
+        l += e.weight * (tc * logf(tc / sc) + (1.0f - tc) * logf((1.0f - tc) / (1.0f - sc)));
+        g[a] = e.weight * (s - tt);
+      }
+    }
+    for (int k = 0; k < npad; ++k) {
+      const bf16 d = (bf16)(g[k] * grad_scale);
+      dout[(size_t)v * npad + k] = d;
+      doutt[(size_t)k * rows + v] = d;
+    }
+  }
+  // block sum of the loss, one atomic per block (global memory, vector path)
+  __shared__ float red[256];
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss_sum, red[0] * loss_scale);
+}
+
+// Raw outputs -> model parameters, entries in order (a ratio entry reads the
+// already-decoded parameter it refers to).  Parameters without an entry keep
+// their value.
+__global__ void decode_kernel(int n, int npad, const float* __restrict__ out, Head h, int nparams,
+                              float* __restrict__ params) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const float* o = out + (size_t)v * npad;
+  float* p = params + (size_t)v * nparams;
+  for (int j = 0; j < h.n; ++j) {
+    const cdm_head_entry& e = h.e[j];
+    const int a = h.out_index[j];
+    if (e.kind == CDM_HEAD_AXIS) {
+      const float vx = o[a], vy = o[a + 1], vz = o[a + 2];
+      const float r = sqrtf(vx * vx + vy * vy + vz * vz);
+      if (r > 0.0f) {
+        p[e.param] = acosf(fminf(fmaxf(vz / r, -1.0f), 1.0f));
+        p[e.param2] = atan2f(vy, vx);
+      }
+    } else {
+      const float s = sigmoid(o[a]);
+      float x;
+      if (e.kind == CDM_HEAD_SIGMOID) x = e.lo + (e.hi - e.lo) * s;
+      else if (e.kind == CDM_HEAD_SIGMOID_LOG) x = expf(logf(e.lo) + (logf(e.hi) - logf(e.lo)) * s);
+      else x = (e.lo + (e.hi - e.lo) * s) * p[e.param2];
+      p[e.param] = fminf(fmaxf(x, e.vmin), e.vmax);
+    }
+  }
+}
+
+// db[n] = sum_b dzt[n][b] (one wave per output unit).
+__global__ void bias_grad_kernel(int nunits, int rows, const bf16* __restrict__ dzt, float* db) {
+  const int unit = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (unit >= nunits) return;
+  float s = 0.0f;
+  for (int b = lane; b < rows; b += 64) s += (float)dzt[(size_t)unit * rows + b];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) db[unit] = s;
+}
+
+// Adam on one weight matrix [nrow][ncol] (padded storage, live region
+// [rows_live][cols_live]) or a bias vector (ncol = 1): FP32 master, moments,
+// and the BF16 copies in both layouts.
+__global__ void adam_kernel(int nrow, int ncol, int rows_live, int cols_live, float* w, const float* g, float* m,
+                            float* v, bf16* wbf, bf16* wbft, float lr, float b1, float b2, float eps, float c1,
+                            float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrow * ncol) return;
+  const int r = i / ncol, c = i - r * ncol;
+  if (r >= rows_live || c >= cols_live) return;  // padding stays zero
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  if (wbf) {
+    const bf16 h = (bf16)wi;
+    wbf[i] = h;
+    wbft[(size_t)c * nrow + r] = h;
+  }
+}
+
+__global__ void to_bf16_kernel(int nrow, int ncol, const float* w, bf16* wbf, bf16* wbft) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrow * ncol) return;
+  const int r = i / ncol, c = i - r * ncol;
+  const bf16 h = (bf16)w[i];
+  wbf[i] = h;
+  wbft[(size_t)c * nrow + r] = h;
+}
+
+dim3 grid1(long long n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ handle
+struct cdm_mlp {
+  int n_in = 0, width = 0, depth = 0, n_out = 0;
+  int kpad = 0, npad = 0;  // padded input width and output width
+  Head head;
+  int nlayers = 0;
+  // per weight layer: out x in (padded), live sizes
+  int rows[kMaxLayers], cols[kMaxLayers], rows_live[kMaxLayers], cols_live[kMaxLayers];
+  float *w[kMaxLayers] = {}, *b[kMaxLayers] = {}, *gw[kMaxLayers] = {}, *gb[kMaxLayers] = {};
+  float *mw[kMaxLayers] = {}, *vw[kMaxLayers] = {}, *mb[kMaxLayers] = {}, *vb[kMaxLayers] = {};
+  bf16 *wbf[kMaxLayers] = {}, *wbft[kMaxLayers] = {};
+  long long adam_step = 0;
+  // workspaces (grown on demand)
+  std::vector<void*> ws;
+  int train_rows = 0, infer_rows = 0;
+  bf16 *x = nullptr, *xt = nullptr;
+  float* z[kMaxLayers] = {};
+  bf16 *h[kMaxLayers] = {}, *ht[kMaxLayers] = {};
+  float* out = nullptr;
+  bf16 *dout = nullptr, *doutt = nullptr, *dz[2] = {}, *dzt[2] = {};
+  float* loss_sum = nullptr;
+  bf16 *ix = nullptr, *ih[2] = {};
+  float* iout = nullptr;
+};
+
+namespace {
+
+int hip_err(hipError_t e, const char* what) {
+  return cdm::set_error(std::string(what) + ": " + hipGetErrorString(e), -2);
+}
+
+template <class T>
+hipError_t dalloc(T** p, size_t count) {
+  return hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T));
+}
+
+void free_all(cdm_mlp* m) {
+  for (int l = 0; l < kMaxLayers; ++l) {
+    for (float* p : {m->w[l], m->b[l], m->gw[l], m->gb[l], m->mw[l], m->vw[l], m->mb[l], m->vb[l], m->z[l]})
+      if (p) (void)hipFree(p);
+    for (bf16* p : {m->wbf[l], m->wbft[l], m->h[l], m->ht[l]})
+      if (p) (void)hipFree(p);
+  }
+  for (void* p : {(void*)m->x, (void*)m->xt, (void*)m->out, (void*)m->dout, (void*)m->doutt, (void*)m->dz[0],
+                  (void*)m->dz[1], (void*)m->dzt[0], (void*)m->dzt[1], (void*)m->loss_sum, (void*)m->ix,
+                  (void*)m->ih[0], (void*)m->ih[1], (void*)m->iout})
+    if (p) (void)hipFree(p);
+}
+
+// splitmix64 -> uniform in [-1, 1)
+struct HostRng {
+  uint64_t s;
+  double next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  }
+};
+
+int upload_layer(cdm_mlp* m, int l, const std::vector<float>& wpad, const std::vector<float>& bpad) {
+  hipError_t e;
+  const size_t nw = (size_t)m->rows[l] * m->cols[l];
+  if ((e = hipMemcpy(m->w[l], wpad.data(), nw * 4, hipMemcpyHostToDevice)) != hipSuccess) return hip_err(e, "weights");
+  if ((e = hipMemcpy(m->b[l], bpad.data(), m->rows[l] * 4, hipMemcpyHostToDevice)) != hipSuccess)
+    return hip_err(e, "weights");
+  hipLaunchKernelGGL(to_bf16_kernel, grid1(nw), dim3(256), 0, 0, m->rows[l], m->cols[l], m->w[l], m->wbf[l],
+                     m->wbft[l]);
+  if ((e = hipGetLastError()) != hipSuccess) return hip_err(e, "weights");
+  if ((e = hipDeviceSynchronize()) != hipSuccess) return hip_err(e, "weights");
+  return 0;
+}
+
+int ensure_train(cdm_mlp* m, int rows) {
+  if (m->train_rows >= rows) return 0;
+  hipError_t e = hipSuccess;
+  for (void* p : {(void*)m->x, (void*)m->xt, (void*)m->out, (void*)m->dout, (void*)m->doutt, (void*)m->dz[0],
+                  (void*)m->dz[1], (void*)m->dzt[0], (void*)m->dzt[1]})
+    if (p) (void)hipFree(p);
+  for (int l = 0; l < kMaxLayers; ++l) {
+    if (m->z[l]) (void)hipFree(m->z[l]);
+    if (m->h[l]) (void)hipFree(m->h[l]);
+    if (m->ht[l]) (void)hipFree(m->ht[l]);
+    m->z[l] = nullptr;
+    m->h[l] = m->ht[l] = nullptr;
+  }
+  const size_t W = m->width;
+  if (e == hipSuccess) e = dalloc(&m->x, (size_t)rows * m->kpad);
+  if (e == hipSuccess) e = dalloc(&m->xt, (size_t)rows * m->kpad);
+  for (int l = 0; l < m->depth && e == hipSuccess; ++l) {
+    e = dalloc(&m->z[l], rows * W);
+    if (e == hipSuccess) e = dalloc(&m->h[l], rows * W);
+    if (e == hipSuccess) e = dalloc(&m->ht[l], rows * W);
+  }
+  if (e == hipSuccess) e = dalloc(&m->out, (size_t)rows * m->npad);
+  if (e == hipSuccess) e = dalloc(&m->dout, (size_t)rows * m->npad);
+  if (e == hipSuccess) e = dalloc(&m->doutt, (size_t)rows * m->npad);
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+    e = dalloc(&m->dz[i], rows * W);
+    if (e == hipSuccess) e = dalloc(&m->dzt[i], rows * W);
+  }
+  if (e != hipSuccess) {
+    m->train_rows = 0;
+    return hip_err(e, "network training workspace");
+  }
+  m->train_rows = rows;
+  return 0;
+}
+
+int ensure_infer(cdm_mlp* m) {
+  if (m->infer_rows) return 0;
+  const int rows = kInferChunk;
+  hipError_t e = dalloc(&m->ix, (size_t)rows * m->kpad);
+  if (e == hipSuccess) e = dalloc(&m->ih[0], (size_t)rows * m->width);
+  if (e == hipSuccess) e = dalloc(&m->ih[1], (size_t)rows * m->width);
+  if (e == hipSuccess) e = dalloc(&m->iout, (size_t)rows * m->npad);
+  if (e != hipSuccess) return hip_err(e, "network inference workspace");
+  m->infer_rows = rows;
+  return 0;
+}
+
+template <int EPI>
+void gemm(const bf16* A, int lda, const bf16* B, int ldb, int M, int N, int K, const Epi& e, hipStream_t s) {
+  hipLaunchKernelGGL(gemm_nt<EPI>, dim3((N + kTile - 1) / kTile, (M + kTile - 1) / kTile), dim3(256), 0, s, A, lda,
+                     B, ldb, M, N, K, e);
+}
+
+Epi epi0() {
+  Epi e;
+  std::memset(&e, 0, sizeof(e));
+  return e;
+}
+
+// Forward over `rows` (multiple of 32) rows of x.  Training keeps z, h, ht.
+void forward_train(cdm_mlp* m, int rows, hipStream_t s) {
+  const bf16* in = m->x;
+  int ld_in = m->kpad, k = m->kpad;
+  for (int l = 0; l < m->depth; ++l) {
+    Epi e = epi0();
+    e.bias = m->b[l];
+    e.out_f32 = m->z[l];
+    e.ld_f32 = m->width;
+    e.out_bf = m->h[l];
+    e.ld_bf = m->width;
+    e.out_bft = m->ht[l];
+    e.ld_bft = rows;
+    gemm<EPI_FWD_GELU>(in, ld_in, m->wbf[l], m->cols[l], rows, m->width, k, e, s);
+    in = m->h[l];
+    ld_in = k = m->width;
+  }
+  Epi e = epi0();
+  e.bias = m->b[m->depth];
+  e.out_f32 = m->out;
+  e.ld_f32 = m->npad;
+  gemm<EPI_PLAIN>(in, ld_in, m->wbf[m->depth], m->width, rows, m->npad, k, e, s);
+}
+
+// Backward from dout/doutt; fills gw, gb.
+void backward(cdm_mlp* m, int rows, hipStream_t s) {
+  const int L = m->depth;
+  // output layer
+  {
+    Epi e = epi0();
+    e.out_f32 = m->gw[L];
+    e.ld_f32 = m->width;
+    gemm<EPI_PLAIN>(m->doutt, rows, m->ht[L - 1], rows, m->npad, m->width, rows, e, s);
+    hipLaunchKernelGGL(bias_grad_kernel, dim3((m->npad + 3) / 4), dim3(256), 0, s, m->npad, rows, m->doutt, m->gb[L]);
+  }
+  // dH_{L-1} = dout W_L, through gelu' -> dz
+  const bf16* dcur = m->dout;
+  int ld_dcur = m->npad, kcur = m->npad;
+  const bf16* wt = m->wbft[L];
+  for (int l = L - 1; l >= 0; --l) {
+    const int slot = l & 1;
+    Epi e = epi0();
+    e.zin = m->z[l];
+    e.ld_zin = m->width;
+    e.out_bf = m->dz[slot];
+    e.ld_bf = m->width;
+    e.out_bft = m->dzt[slot];
+    e.ld_bft = rows;
+    gemm<EPI_BWD_GELU>(dcur, ld_dcur, wt, kcur, rows, m->width, kcur, e, s);
+    // dW_l = dz^T h_{l-1}
+    Epi g = epi0();
+    g.out_f32 = m->gw[l];
+    g.ld_f32 = m->cols[l];
+    const bf16* hin_t = l > 0 ? m->ht[l - 1] : m->xt;
+    gemm<EPI_PLAIN>(m->dzt[slot], rows, hin_t, rows, m->width, m->cols[l], rows, g, s);
+    hipLaunchKernelGGL(bias_grad_kernel, dim3((m->width + 3) / 4), dim3(256), 0, s, m->width, rows, m->dzt[slot],
+                       m->gb[l]);
+    dcur = m->dz[slot];
+    ld_dcur = kcur = m->width;
+    wt = m->wbft[l];
+  }
+}
+
+void adam(cdm_mlp* m, float lr, hipStream_t s) {
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  ++m->adam_step;
+  const float c1 = 1.0f - std::pow(b1, (float)m->adam_step), c2 = 1.0f - std::pow(b2, (float)m->adam_step);
+  for (int l = 0; l <= m->depth; ++l) {
+    const int n = m->rows[l] * m->cols[l];
+    hipLaunchKernelGGL(adam_kernel, grid1(n), dim3(256), 0, s, m->rows[l], m->cols[l], m->rows_live[l],
+                       m->cols_live[l], m->w[l], m->gw[l], m->mw[l], m->vw[l], m->wbf[l], m->wbft[l], lr, b1, b2, eps,
+                       c1, c2);
+    hipLaunchKernelGGL(adam_kernel, grid1(m->rows[l]), dim3(256), 0, s, m->rows[l], 1, m->rows_live[l], 1, m->b[l],
+                       m->gb[l], m->mb[l], m->vb[l], (bf16*)nullptr, (bf16*)nullptr, lr, b1, b2, eps, c1, c2);
+  }
+}
+
+// Inference over n voxels in chunks: raw outputs to out [n][npad] (device,
+// caller's) or straight into the decode step.
+int infer(cdm_mlp* m, int n, int nmeas, const float* meas, const float* scale, float* out_raw, int nparams,
+          float* params, hipStream_t s) {
+  int r = ensure_infer(m);
+  if (r) return r;
+  for (int first = 0; first < n; first += kInferChunk) {
+    const int cnt = std::min(kInferChunk, n - first);
+    const int rows = round_up(cnt, 32);
+    hipLaunchKernelGGL(inputs_kernel, grid1((long long)rows * m->kpad), dim3(256), 0, s, cnt, rows, nmeas, m->kpad,
+                       meas + (size_t)first * nmeas, scale ? scale + first : nullptr, m->ix, (bf16*)nullptr);
+    const bf16* in = m->ix;
+    int k = m->kpad;
+    for (int l = 0; l < m->depth; ++l) {
+      Epi e = epi0();
+      e.bias = m->b[l];
+      e.out_bf = m->ih[l & 1];
+      e.ld_bf = m->width;
+      gemm<EPI_FWD_GELU>(in, k, m->wbf[l], m->cols[l], rows, m->width, k, e, s);
+      in = m->ih[l & 1];
+      k = m->width;
+    }
+    Epi e = epi0();
+    e.bias = m->b[m->depth];
+    e.out_f32 = m->iout;
+    e.ld_f32 = m->npad;
+    gemm<EPI_PLAIN>(in, k, m->wbf[m->depth], m->width, rows, m->npad, k, e, s);
+    if (out_raw) {
+      hipError_t e2 = hipMemcpy2DAsync(out_raw + (size_t)first * m->n_out, m->n_out * sizeof(float), m->iout,
+                                       m->npad * sizeof(float), m->n_out * sizeof(float), cnt, hipMemcpyDeviceToDevice,
+                                       s);
+      if (e2 != hipSuccess) return hip_err(e2, "network outputs");
+    }
+    if (params)
+      hipLaunchKernelGGL(decode_kernel, grid1(cnt), dim3(256), 0, s, cnt, m->npad, m->iout, m->head, nparams,
+                         params + (size_t)first * nparams);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : hip_err(e, "network inference launch");
+}
+
+int check_params(const cdm_mlp* m, int nparams) {
+  for (int j = 0; j < m->head.n; ++j) {
+    const cdm_head_entry& e = m->head.e[j];
+    if (e.param >= nparams || ((e.kind == CDM_HEAD_AXIS || e.kind == CDM_HEAD_SIGMOID_RATIO) && e.param2 >= nparams))
+      return cdm::set_error("network head refers to a parameter beyond nparams");
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cdm_mlp_create(int n_in, int width, int depth, const cdm_head_entry* head, int nhead, unsigned long long seed,
+                   cdm_mlp** out) {
+  if (!out) return cdm::set_error("null output handle");
+  *out = nullptr;
+  if (n_in <= 0 || width <= 0 || width % 32 != 0 || depth < 1 || depth > kMaxLayers - 1)
+    return cdm::set_error("network shape: n_in > 0, width a positive multiple of 32, 1 <= depth <= 7");
+  if (!head || nhead <= 0 || nhead > kMaxHead) return cdm::set_error("network head: 1..16 entries");
+  cdm_mlp* m = new cdm_mlp();
+  m->n_in = n_in;
+  m->width = width;
+  m->depth = depth;
+  m->head.n = nhead;
+  int n_out = 0;
+  for (int j = 0; j < nhead; ++j) {
+    const cdm_head_entry& e = head[j];
+    if (e.kind < CDM_HEAD_SIGMOID || e.kind > CDM_HEAD_AXIS || e.param < 0 ||
+        ((e.kind == CDM_HEAD_AXIS || e.kind == CDM_HEAD_SIGMOID_RATIO) && e.param2 < 0) ||
+        (e.kind != CDM_HEAD_AXIS && !(e.hi > e.lo)) || (e.kind == CDM_HEAD_SIGMOID_LOG && !(e.lo > 0))) {
+      delete m;
+      return cdm::set_error("network head entry " + std::to_string(j) + " is invalid");
+    }
+    m->head.e[j] = e;
+    m->head.out_index[j] = n_out;
+    n_out += e.kind == CDM_HEAD_AXIS ? 3 : 1;
+  }
+  if (n_out > 32) {
+    delete m;
+    return cdm::set_error("network head: at most 32 outputs");
+  }
+  m->n_out = n_out;
+  m->kpad = round_up(n_in, 32);
+  m->npad = 32;
+  m->nlayers = depth + 1;
+  HostRng rng{seed};
+  for (int l = 0; l <= depth; ++l) {
+    m->rows[l] = l == depth ? m->npad : width;
+    m->cols[l] = l == 0 ? m->kpad : width;
+    m->rows_live[l] = l == depth ? n_out : width;
+    m->cols_live[l] = l == 0 ? n_in : width;
+    const size_t nw = (size_t)m->rows[l] * m->cols[l];
+    hipError_t e = hipSuccess;
+    for (float** p : {&m->w[l], &m->gw[l], &m->mw[l], &m->vw[l]})
+      if (e == hipSuccess) e = dalloc(p, nw);
+    for (float** p : {&m->b[l], &m->gb[l], &m->mb[l], &m->vb[l]})
+      if (e == hipSuccess) e = dalloc(p, m->rows[l]);
+    if (e == hipSuccess) e = dalloc(&m->wbf[l], nw);
+    if (e == hipSuccess) e = dalloc(&m->wbft[l], nw);
+    for (float* p : {m->mw[l], m->vw[l]})
+      if (e == hipSuccess) e = hipMemset(p, 0, nw * 4);
+    for (float* p : {m->mb[l], m->vb[l], m->gb[l]})
+      if (e == hipSuccess) e = hipMemset(p, 0, m->rows[l] * 4);
+    if (e != hipSuccess) {
+      free_all(m);
+      delete m;
+      return hip_err(e, "network weights");
+    }
+    // LeCun-uniform initialisation: variance 1 / fan_in
+    std::vector<float> w(nw, 0.0f), b(m->rows[l], 0.0f);
+    const double a = std::sqrt(3.0 / m->cols_live[l]);
+    for (int r = 0; r < m->rows_live[l]; ++r)
+      for (int c = 0; c < m->cols_live[l]; ++c) w[(size_t)r * m->cols[l] + c] = (float)(a * rng.next());
+    const int rc = upload_layer(m, l, w, b);
+    if (rc) {
+      free_all(m);
+      delete m;
+      return rc;
+    }
+  }
+  *out = m;
+  return 0;
+}
+
+void cdm_mlp_destroy(cdm_mlp* m) {
+  if (!m) return;
+  (void)hipDeviceSynchronize();
+  free_all(m);
+  delete m;
+}
+
+int cdm_mlp_info(const cdm_mlp* m, int* n_in, int* width, int* depth, int* n_out, long long* nweights) {
+  if (!m) return cdm::set_error("null network handle");
+  long long nw = 0;
+  for (int l = 0; l <= m->depth; ++l) nw += (long long)m->rows_live[l] * m->cols_live[l] + m->rows_live[l];
+  if (n_in) *n_in = m->n_in;
+  if (width) *width = m->width;
+  if (depth) *depth = m->depth;
+  if (n_out) *n_out = m->n_out;
+  if (nweights) *nweights = nw;
+  return 0;
+}
+
+// Flat host layout: for each layer, W[out][in] (live part) then b[out].
+int cdm_mlp_get_weights(const cdm_mlp* m, float* w) {
+  if (!m || !w) return cdm::set_error("null argument");
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_err(e, "network weights");
+  for (int l = 0; l <= m->depth; ++l) {
+    std::vector<float> wp((size_t)m->rows[l] * m->cols[l]), bp(m->rows[l]);
+    if ((e = hipMemcpy(wp.data(), m->w[l], wp.size() * 4, hipMemcpyDeviceToHost)) != hipSuccess)
+      return hip_err(e, "network weights");
+    if ((e = hipMemcpy(bp.data(), m->b[l], bp.size() * 4, hipMemcpyDeviceToHost)) != hipSuccess)
+      return hip_err(e, "network weights");
+    for (int r = 0; r < m->rows_live[l]; ++r)
+      for (int c = 0; c < m->cols_live[l]; ++c) *w++ = wp[(size_t)r * m->cols[l] + c];
+    for (int r = 0; r < m->rows_live[l]; ++r) *w++ = bp[r];
+  }
+  return 0;
+}
+
+int cdm_mlp_set_weights(cdm_mlp* m, const float* w) {
+  if (!m || !w) return cdm::set_error("null argument");
+  for (int l = 0; l <= m->depth; ++l) {
+    std::vector<float> wp((size_t)m->rows[l] * m->cols[l], 0.0f), bp(m->rows[l], 0.0f);
+    for (int r = 0; r < m->rows_live[l]; ++r)
+      for (int c = 0; c < m->cols_live[l]; ++c) wp[(size_t)r * m->cols[l] + c] = *w++;
+    for (int r = 0; r < m->rows_live[l]; ++r) bp[r] = *w++;
+    const int rc = upload_layer(m, l, wp, bp);
+    if (rc) return rc;
+  }
+  m->adam_step = 0;
+  for (int l = 0; l <= m->depth; ++l) {
+    const size_t nw = (size_t)m->rows[l] * m->cols[l];
+    hipError_t e = hipSuccess;
+    for (float* p : {m->mw[l], m->vw[l]})
+      if (e == hipSuccess) e = hipMemset(p, 0, nw * 4);
+    for (float* p : {m->mb[l], m->vb[l]})
+      if (e == hipSuccess) e = hipMemset(p, 0, m->rows[l] * 4);
+    if (e != hipSuccess) return hip_err(e, "network optimiser state");
+  }
+  return 0;
+}
+
+int cdm_mlp_forward(cdm_mlp* m, int n, int nmeas, const float* meas, const float* scale, float* out, void* stream) {
+  if (!m) return cdm::set_error("null network handle");
+  if (n < 0 || nmeas != m->n_in) return cdm::set_error("nmeas must equal the network's input width");
+  if (n == 0) return 0;
+  if (!meas || !out) return cdm::set_error("null device pointer");
+  return infer(m, n, nmeas, meas, scale, out, 0, nullptr, (hipStream_t)stream);
+}
+
+int cdm_mlp_init_params(cdm_mlp* m, int n, int nmeas, const float* meas, const float* scale, int nparams,
+                        float* params, void* stream) {
+  if (!m) return cdm::set_error("null network handle");
+  if (n < 0 || nmeas != m->n_in) return cdm::set_error("nmeas must equal the network's input width");
+  int r = check_params(m, nparams);
+  if (r) return r;
+  if (n == 0) return 0;
+  if (!meas || !params) return cdm::set_error("null device pointer");
+  return infer(m, n, nmeas, meas, scale, nullptr, nparams, params, (hipStream_t)stream);
+}
+
+int cdm_mlp_loss_grad(cdm_mlp* m, int n, int nmeas, const float* meas, const float* scale, int nparams,
+                      const float* params_true, float* loss, float* grad, void* stream) {
+  if (!m) return cdm::set_error("null network handle");
+  if (n <= 0 || n % 32 != 0 || nmeas != m->n_in)
+    return cdm::set_error("loss_grad: n a positive multiple of 32, nmeas equal to the input width");
+  int r = check_params(m, nparams);
+  if (r) return r;
+  if (!meas || !params_true || !loss || !grad) return cdm::set_error("null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if ((r = ensure_train(m, n))) return r;
+  float* target = nullptr;
+  hipError_t e = dalloc(&target, (size_t)n * m->n_out);
+  if (e == hipSuccess && !m->loss_sum) e = dalloc(&m->loss_sum, 1);
+  if (e != hipSuccess) return hip_err(e, "loss_grad workspace");
+  hipLaunchKernelGGL(encode_kernel, grid1(n), dim3(256), 0, s, n, nparams, params_true, m->head, m->n_out, target);
+  hipLaunchKernelGGL(inputs_kernel, grid1((long long)n * m->kpad), dim3(256), 0, s, n, n, nmeas, m->kpad, meas, scale,
+                     m->x, m->xt);
+  forward_train(m, n, s);
+  (void)hipMemsetAsync(m->loss_sum, 0, sizeof(float), s);
+  hipLaunchKernelGGL(loss_kernel, grid1(n), dim3(256), 0, s, n, m->n_out, m->npad, m->out, target, m->head,
+                     1.0f / n, 1.0f / n, m->dout, m->doutt, m->loss_sum);
+  backward(m, n, s);
+  e = hipStreamSynchronize(s);
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(loss, m->loss_sum, sizeof(float), hipMemcpyDeviceToHost);
+  for (int l = 0; l <= m->depth && e == hipSuccess; ++l) {
+    std::vector<float> gp((size_t)m->rows[l] * m->cols[l]), bp(m->rows[l]);
+    e = hipMemcpy(gp.data(), m->gw[l], gp.size() * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(bp.data(), m->gb[l], bp.size() * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) break;
+    for (int rr = 0; rr < m->rows_live[l]; ++rr)
+      for (int c = 0; c < m->cols_live[l]; ++c) *grad++ = gp[(size_t)rr * m->cols[l] + c];
+    for (int rr = 0; rr < m->rows_live[l]; ++rr) *grad++ = bp[rr];
+  }
+  (void)hipFree(target);
+  return e == hipSuccess ? 0 : hip_err(e, "loss_grad");
+}
+
+int cdm_mlp_train(cdm_mlp* m, int n, int nmeas, const float* clean, const float* scale, int nparams,
+                  const float* params_true, float snr_lo, float snr_hi, int epochs, int batch, float lr,
+                  unsigned long long seed, float* epoch_loss, void* stream) {
+  if (!m) return cdm::set_error("null network handle");
+  if (nmeas != m->n_in) return cdm::set_error("nmeas must equal the network's input width");
+  if (batch <= 0 || batch % 32 != 0) return cdm::set_error("batch must be a positive multiple of 32");
+  if (n < batch) return cdm::set_error("fewer training samples than one batch");
+  if (epochs < 0 || !(snr_hi >= snr_lo) || !(snr_lo > 0) || !(lr > 0)) return cdm::set_error("bad training options");
+  int r = check_params(m, nparams);
+  if (r) return r;
+  if (!clean || !params_true) return cdm::set_error("null device pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if ((r = ensure_train(m, batch))) return r;
+  float* target = nullptr;
+  hipError_t e = dalloc(&target, (size_t)n * m->n_out);
+  if (e == hipSuccess && !m->loss_sum) e = dalloc(&m->loss_sum, 1);
+  if (e != hipSuccess) return hip_err(e, "training workspace");
+  hipLaunchKernelGGL(encode_kernel, grid1(n), dim3(256), 0, s, n, nparams, params_true, m->head, m->n_out, target);
+  const int nb = n / batch;  // whole batches; the remainder rotates in through the epoch offset
+  const long long total = (long long)nb * epochs;
+  long long step = 0;
+  for (int ep = 0; ep < epochs && e == hipSuccess; ++ep) {
+    (void)hipMemsetAsync(m->loss_sum, 0, sizeof(float), s);
+    const long long shift = (long long)(ep * 7919LL % std::max(1, n - nb * batch + 1));
+    for (int bi = 0; bi < nb; ++bi, ++step) {
+      const long long first = shift + (long long)bi * batch;
+      hipLaunchKernelGGL(noisy_inputs_kernel, grid1((long long)batch * m->kpad), dim3(256), 0, s, batch, nmeas,
+                         m->kpad, first, clean, scale, snr_lo, snr_hi, (uint64_t)seed, (uint32_t)ep, m->x, m->xt);
+      forward_train(m, batch, s);
+      hipLaunchKernelGGL(loss_kernel, grid1(batch), dim3(256), 0, s, batch, m->n_out, m->npad, m->out,
+                         target + (size_t)first * m->n_out, m->head, 1.0f / batch, 1.0f / (batch * (float)nb),
+                         m->dout, m->doutt, m->loss_sum);
+      backward(m, batch, s);
+      const float cosw = 0.5f * (1.0f + std::cos(3.14159265f * (float)step / (float)std::max(1LL, total)));
+      adam(m, lr * (0.02f + 0.98f * cosw), s);
+    }
+    e = hipGetLastError();
+    if (e == hipSuccess && epoch_loss) e = hipMemcpyAsync(&epoch_loss[ep], m->loss_sum, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(target);
+  return e == hipSuccess ? 0 : hip_err(e, "network training");
+}
+
+}  // extern "C"

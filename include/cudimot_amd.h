@@ -120,6 +120,73 @@ int cdm_bic_aic(cdm_model* m, int nvox, int nmeas, int nsamples, int rician, con
                 const float* samples, const float* tau_samples, const float* cfp, const float* fixp, float* bic,
                 float* aic, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Learned initialiser.  No counterpart in the reference: it stands in for
+ * GridSearch (GridSearch.cu:291-317) as the source of LM starting points.
+ * A fully connected network n_in -> width (x depth, GELU) -> n_out, with
+ * BF16 MFMA products and FP32 accumulation for inference and training.
+ * Inputs are a voxel's measurements divided by a per-voxel scale (S0).
+ * The output head maps network outputs to model parameters, one entry per
+ * predicted parameter (or parameter pair, for an axis):
+ *   CDM_HEAD_SIGMOID        p = lo + (hi - lo) s(o)                 1 output
+ *   CDM_HEAD_SIGMOID_LOG    p = exp(log lo + (log hi - log lo) s(o)) 1 output
+ *   CDM_HEAD_SIGMOID_RATIO  p = (lo + (hi - lo) s(o)) P[param2]       1 output
+ *   CDM_HEAD_AXIS           (P[param], P[param2]) = (theta, phi) of the
+ *                           output vector (o0, o1, o2)               3 outputs
+ * with s the logistic function; decoded values are clamped to [vmin, vmax]
+ * (axis entries excepted).  Training loss: weight x soft-target cross entropy
+ * per sigmoid entry, weight x (1 - (v.n)^2 / |v|^2) per axis. */
+typedef struct cdm_mlp cdm_mlp;
+
+enum cdm_head_kind { CDM_HEAD_SIGMOID = 0, CDM_HEAD_SIGMOID_LOG = 1, CDM_HEAD_SIGMOID_RATIO = 2, CDM_HEAD_AXIS = 3 };
+
+typedef struct cdm_head_entry {
+  int kind;    /* cdm_head_kind */
+  int param;   /* parameter index (theta for an axis) */
+  int param2;  /* ratio reference, or phi for an axis; -1 otherwise */
+  float lo, hi;
+  float weight;     /* loss weight */
+  float vmin, vmax; /* clamp of the decoded value */
+} cdm_head_entry;
+
+/* Create a network with LeCun-uniform weights from `seed`; head is a host
+ * array of nhead entries (decoded in order). */
+int cdm_mlp_create(int n_in, int width, int depth, const cdm_head_entry* head, int nhead, unsigned long long seed,
+                   cdm_mlp** out);
+void cdm_mlp_destroy(cdm_mlp* m);
+int cdm_mlp_info(const cdm_mlp* m, int* n_in, int* width, int* depth, int* n_out, long long* nweights);
+/* Host copy of the FP32 master weights, per layer W[out][in] then b[out]. */
+int cdm_mlp_get_weights(const cdm_mlp* m, float* w);
+/* Replace the weights (resets the optimiser state). */
+int cdm_mlp_set_weights(cdm_mlp* m, const float* w);
+
+/* Raw network outputs for n voxels: out [n][n_out] (device).  nmeas must be
+ * the network's input width; scale (device float[n]) may be NULL (1). */
+int cdm_mlp_forward(cdm_mlp* m, int n, int nmeas, const float* meas, const float* scale, float* out, void* stream);
+
+/* Learned starting point: writes the head's parameters into params
+ * [n][nparams] (device); parameters without a head entry are left as they
+ * are (the caller fills them, e.g. with P_init). */
+int cdm_mlp_init_params(cdm_mlp* m, int n, int nmeas, const float* meas, const float* scale, int nparams,
+                        float* params, void* stream);
+
+/* One forward + backward pass over n (multiple of 32) voxels of given
+ * measurements without updating: mean loss and its gradient with respect to
+ * every weight (host arrays, get_weights layout).  params_true [n][nparams]
+ * (device) are the targets.  For gradient checks. */
+int cdm_mlp_loss_grad(cdm_mlp* m, int n, int nmeas, const float* meas, const float* scale, int nparams,
+                      const float* params_true, float* loss, float* grad, void* stream);
+
+/* Train with Adam (cosine-decayed learning rate) on n noise-free signals
+ * clean [n][nmeas] (device) of known parameters params_true [n][nparams];
+ * every epoch draws fresh Rician noise, sigma = scale / SNR with SNR uniform
+ * in [snr_lo, snr_hi] per sample (Philox streams under `seed`).  batch is a
+ * multiple of 32.  epoch_loss (host, [epochs], nullable) receives each
+ * epoch's mean loss. */
+int cdm_mlp_train(cdm_mlp* m, int n, int nmeas, const float* clean, const float* scale, int nparams,
+                  const float* params_true, float snr_lo, float snr_hi, int epochs, int batch, float lr,
+                  unsigned long long seed, float* epoch_loss, void* stream);
+
 /* Host-side data preparation for --rician: per voxel, replace non-positive
  * signals by the smallest positive one when some are positive
  * (dMRI_Data.cu:14-31).  meas is HOST memory [nvox][nmeas]. */
