@@ -106,6 +106,8 @@ def build(models=None, extra_dirs=(), force=False, jobs=None):
         jobs_list.append((os.path.join(CSRC, "host", f), os.path.join(BUILD, f[:-4] + ".o"), host_deps))
     jobs_list.append((os.path.join(CSRC, "mlp", "mlp.hip"), os.path.join(BUILD, "mlp.o"),
                       host_deps + [os.path.join(CSRC, "mlp"), os.path.join(CSRC, "engine", "philox.h")]))
+    jobs_list.append((os.path.join(CSRC, "init", "orientation.hip"), os.path.join(BUILD, "orientation.o"),
+                      host_deps + [os.path.join(CSRC, "init")]))
     if force:
         for _, o, _ in jobs_list:
             if os.path.exists(o + ".sha"):

@@ -24,10 +24,12 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise CdmError("engine library not built: %s (run python -m cudimot_amd.build)" % LIB_PATH)
-        # PyTorch-ROCm wheels carry their own HIP/HSA runtime next to the one
-        # this library links (/opt/rocm).  Both share the process's GPU
-        # address space; the device is found by both only when torch's
-        # runtime is loaded first, so load it before this library.
+        # PyTorch-ROCm wheels carry their own HIP/HSA runtime with the same
+        # sonames as /opt/rocm's (libamdhip64.so.7, libhsa-runtime64.so.1).
+        # Loading torch first makes this library bind to that one runtime, so
+        # torch tensors, streams and events and this library's kernels share
+        # queues and ordering.  Loaded the other way round, two runtimes
+        # coexist in the process and the second one finds no device.
         try:
             import torch  # noqa: F401
         except ImportError:
@@ -47,7 +49,7 @@ def lib():
             "cdm_model_load_grid": (i, [vp, cp, i, ip, ip]),
             "cdm_model_grid": (i, [vp, ip, dp]),
             "cdm_model_set_group": (i, [vp, i, i]),
-            "cdm_model_get_group": (i, [vp, i, i]),
+            "cdm_model_get_group": (i, [vp, i, i, i]),
             "cdm_model_load_tuning": (i, [vp, cp]),
             "cdm_model_set_mcmc_blocks": (i, [vp, i, i]),
             "cdm_gridsearch": (i, [vp, i, i, vp, vp, vp, vp, vp, vp]),
@@ -56,6 +58,7 @@ def lib():
             "cdm_predicted_signal": (i, [vp, i, i, i, vp, vp, vp, vp, vp]),
             "cdm_bic_aic": (i, [vp, i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp]),
             "cdm_rician_clean": (i, [i, i, vp]),
+            "cdm_tensor_frame": (i, [i, vp, i, i, vp, vp]),
             "cdm_mlp_create": (i, [i, i, i, vp, i, u64, ctypes.POINTER(vp)]),
             "cdm_mlp_destroy": (None, [vp]),
             "cdm_mlp_info": (i, [vp, ip, ip, ip, ip, ctypes.POINTER(i64)]),
@@ -153,13 +156,14 @@ class Model:
         gp = (ctypes.c_int * max(ng.value, 1))()
         g = (ctypes.c_double * max(nc.value * ng.value, 1))()
         check(lib().cdm_model_grid(self.h, gp, g))
-        return list(gp)[:ng.value], [list(g)[k * ng.value:(k + 1) * ng.value] for k in range(nc.value)]
+        flat = list(g)
+        return list(gp)[:ng.value], [flat[k * ng.value:(k + 1) * ng.value] for k in range(nc.value)]
 
     def set_group(self, method, lanes):
         check(lib().cdm_model_set_group(self.h, method, lanes))
 
-    def get_group(self, method, nmeas):
-        r = lib().cdm_model_get_group(self.h, method, nmeas)
+    def get_group(self, method, nmeas, nvox=0):
+        r = lib().cdm_model_get_group(self.h, method, nmeas, int(nvox))
         if r < 0:
             raise CdmError(lib().cdm_last_error().decode())
         return r
@@ -203,6 +207,12 @@ class Model:
         ns = samples.shape[2]
         check(lib().cdm_bic_aic(self.h, nvox, nmeas, ns, 1 if rician else 0, ptr(meas), ptr(samples),
                                 ptr(tau_samples), ptr(cfp), ptr(fixp), ptr(bic), ptr(aic), _stream(stream)))
+
+
+def tensor_frame(dti_params, out, first=1, stream=None):
+    """theta, phi, psi and eigenvalues [n][6] from tensor parameters [n][stride]."""
+    n, stride = dti_params.shape
+    check(lib().cdm_tensor_frame(n, ptr(dti_params), stride, first, ptr(out), _stream(stream)))
 
 
 def sync(stream=None):

@@ -97,8 +97,14 @@ int default_group(int method, int nmeas) {
 }
 
 // Explicit choice, else the tuned entry of the nearest measurement count
-// (log ratio), else the built-in rule.
-int group_for(const cdm_model* m, int method, int nmeas) {
+// (log ratio), else the built-in rule.  Tuned and built-in choices get an
+// occupancy floor: a launch over few voxels widens the group (x4 steps, up to
+// 64 lanes) until the grid covers about two waves per SIMD of the GPU, since
+// a narrow group over a small volume leaves most CUs idle and each voxel's
+// work serial.
+constexpr long long kLaneSlots = 256LL * 4 * 64 * 2;
+
+int group_for(const cdm_model* m, int method, int nmeas, int nvox) {
   if (m->group[method]) return m->group[method];
   int best = 0;
   double dist = 0;
@@ -110,7 +116,9 @@ int group_for(const cdm_model* m, int method, int nmeas) {
       dist = d;
     }
   }
-  return best ? best : default_group(method, nmeas);
+  int L = best ? best : default_group(method, nmeas);
+  while (nvox > 0 && L < 64 && (long long)nvox * L < kLaneSlots) L *= 4;
+  return L;
 }
 
 const char* const kMethodNames[5] = {"gridsearch", "levmar", "mcmc", "predicted", "bic_aic"};
@@ -371,9 +379,9 @@ int cdm_model_load_tuning(cdm_model* m, const char* path) {
   return n < 0 ? fail(err) : n;
 }
 
-int cdm_model_get_group(const cdm_model* m, int method, int nmeas) {
+int cdm_model_get_group(const cdm_model* m, int method, int nmeas, int nvox) {
   if (!m || method < 0 || method > 4) return fail("bad arguments");
-  return group_for(m, method, nmeas);
+  return group_for(m, method, nmeas, nvox);
 }
 
 int cdm_model_set_mcmc_blocks(cdm_model* m, int jumps_per_launch, int use_graph) {
@@ -393,7 +401,7 @@ int cdm_gridsearch(cdm_model* m, int nvox, int nmeas, const float* meas, float* 
   if (nvox == 0) return 0;
   const cdm::Geom g = make_geom(m, nvox, nmeas, meas, cfp, fixp);
   const cdm::Consts c = make_consts(m);
-  const hipError_t e = m->ops->gridsearch(group_for(m, cdm::kGridSearch, nmeas), g, params,
+  const hipError_t e = m->ops->gridsearch(group_for(m, cdm::kGridSearch, nmeas, nvox), g, params,
                                           (int)m->spec.grid_params.size(), m->spec.grid_params.data(),
                                           m->ncombs_valid, m->d_grid_f, m->d_grid_d, m->d_comb_id, c, kept,
                                           (hipStream_t)stream);
@@ -407,7 +415,7 @@ int cdm_levmar(cdm_model* m, int nvox, int nmeas, const float* meas, float* para
   if (max_iters < 0) return fail("--iterLevMar must be >= 0");
   if (nvox == 0) return 0;
   const cdm::Geom g = make_geom(m, nvox, nmeas, meas, cfp, fixp);
-  const hipError_t e = m->ops->levmar(group_for(m, cdm::kLevMar, nmeas), g, params, max_iters, marquardt ? 1 : 0,
+  const hipError_t e = m->ops->levmar(group_for(m, cdm::kLevMar, nmeas, nvox), g, params, max_iters, marquardt ? 1 : 0,
                                       make_consts(m), iters, cost, (hipStream_t)stream);
   return e == hipSuccess ? 0 : hip_fail(e, "Levenberg-Marquardt launch");
 }
@@ -431,7 +439,7 @@ int cdm_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params
   if ((r = ensure_state(m, m->ops->chain_bytes * (size_t)nvox))) return r;
   const cdm::Geom g = make_geom(m, nvox, nmeas, meas, cfp, fixp);
   const cdm::Consts c = make_consts(m);
-  const int L = group_for(m, cdm::kMcmc, nmeas);
+  const int L = group_for(m, cdm::kMcmc, nmeas, nvox);
   hipStream_t s = (hipStream_t)stream;
 
   cdm::McmcLaunch ml;
@@ -492,7 +500,7 @@ int cdm_predicted_signal(cdm_model* m, int nvox, int nmeas, int nsamples, const 
   if (nvox == 0) return 0;
   const cdm::Geom g = make_geom(m, nvox, nmeas, nullptr, cfp, fixp);
   const hipError_t e =
-      m->ops->predicted(group_for(m, cdm::kPredicted, nmeas), g, nsamples, samples, out, (hipStream_t)stream);
+      m->ops->predicted(group_for(m, cdm::kPredicted, nmeas, nvox), g, nsamples, samples, out, (hipStream_t)stream);
   return e == hipSuccess ? 0 : hip_fail(e, "predicted-signal launch");
 }
 
@@ -504,7 +512,7 @@ int cdm_bic_aic(cdm_model* m, int nvox, int nmeas, int nsamples, int rician, con
   if (rician && !tau_samples && nvox > 0) return fail("Rician BIC/AIC needs tau samples");
   if (nvox == 0) return 0;
   const cdm::Geom g = make_geom(m, nvox, nmeas, meas, cfp, fixp);
-  const hipError_t e = m->ops->bic_aic(group_for(m, cdm::kBicAic, nmeas), rician ? 1 : 0, g, nsamples, samples,
+  const hipError_t e = m->ops->bic_aic(group_for(m, cdm::kBicAic, nmeas, nvox), rician ? 1 : 0, g, nsamples, samples,
                                        tau_samples, bic, aic, (hipStream_t)stream);
   return e == hipSuccess ? 0 : hip_fail(e, "BIC/AIC launch");
 }

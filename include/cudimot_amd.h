@@ -66,9 +66,13 @@ int cdm_model_load_grid(cdm_model* m, const char* src, int is_text, int* ncombs,
 int cdm_model_grid(const cdm_model* m, int* grid_params, double* grid);
 
 /* Voxel-group size (lanes per voxel: 1, 4, 16 or 64) used by a method;
- * 0 restores the built-in choice.  The autotuner sets these. */
+ * 0 restores the automatic choice: the tuned or built-in size for the
+ * measurement count, widened for launches over few voxels (nvox x lanes
+ * below ~131k lane slots) so that the grid still fills the GPU.  get_group
+ * reports the size a launch of nvox voxels would use (nvox <= 0: no
+ * widening). */
 int cdm_model_set_group(cdm_model* m, int method, int lanes);
-int cdm_model_get_group(const cdm_model* m, int method, int nmeas);
+int cdm_model_get_group(const cdm_model* m, int method, int nmeas, int nvox);
 
 /* Autotuned launch table (no counterpart in the reference, whose launch
  * shape is fixed: one warp per voxel, 64 voxels per block, MCMC.cu:780-784).
@@ -186,6 +190,15 @@ int cdm_mlp_loss_grad(cdm_mlp* m, int n, int nmeas, const float* meas, const flo
 int cdm_mlp_train(cdm_mlp* m, int n, int nmeas, const float* clean, const float* scale, int nparams,
                   const float* params_true, float snr_lo, float snr_hi, int epochs, int batch, float lr,
                   unsigned long long seed, float* epoch_loss, void* stream);
+
+/* Orientation initialisation from a tensor fit (the reference pipeline's
+ * dtifit V1/V2 -> cart2spherical -> initialise_Psi steps: utils/Run_dtifit.sh:36-46,
+ * utils/cart2spherical.cc:17-69, utils/initialise_Psi.cc:18-106).  dti
+ * [n][stride] (device) holds the tensor (Dxx, Dxy, Dxz, Dyy, Dyz, Dzz) at
+ * column `first` (DiffusionTensor plug-in: first = 1).  out [n][6] (device):
+ * theta, phi of the principal eigenvector, psi of the second eigenvector in
+ * the Bingham frame, and the eigenvalues in descending order. */
+int cdm_tensor_frame(int n, const float* dti, int stride, int first, float* out, void* stream);
 
 /* Host-side data preparation for --rician: per voxel, replace non-positive
  * signals by the smallest positive one when some are positive

@@ -93,8 +93,14 @@ def test_tuning_table_nearest_measurement_count(tmp_path):
     assert m.get_group(capi.MCMC, 99) == 4
     assert m.get_group(capi.MCMC, 120) == 4
     assert m.get_group(capi.MCMC, 250) == 16
+    # occupancy floor for small launches; explicit choices are kept as set
+    assert m.get_group(capi.MCMC, 99, 500000) == 4
+    assert m.get_group(capi.MCMC, 99, 16384) == 16
+    assert m.get_group(capi.MCMC, 99, 100) == 64
     m.set_group(capi.MCMC, 64)
     assert m.get_group(capi.MCMC, 99) == 64
+    m.set_group(capi.MCMC, 1)
+    assert m.get_group(capi.MCMC, 99, 100) == 1
     m.set_group(capi.MCMC, 0)
     assert m.get_group(capi.MCMC, 99) == 4
     t.write_text("NODDI_Bingham 99 levmar 3\n")

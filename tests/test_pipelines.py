@@ -1,0 +1,125 @@
+"""Fitting recipes: orientation from the tensor fit, the reference's
+GridSearch + LM recipe for NODDI-Bingham, and the learned start with its
+GridSearch fallback."""
+import numpy as np
+import pytest
+
+from tests.helpers import dev, host
+
+
+def fan_dir(th, ph, psi):
+    """Fanning direction of the Bingham frame (NODDI_Bingham/modelfunctions.h:99-121)."""
+    return np.stack([-np.cos(psi) * np.sin(ph) - np.cos(th) * np.cos(ph) * np.sin(psi),
+                     np.cos(ph) * np.cos(psi) - np.cos(th) * np.sin(ph) * np.sin(psi),
+                     np.sin(th) * np.sin(psi)], -1)
+
+
+def test_fan_direction_is_orthonormal_to_fibre():
+    """The frame psi is measured in: fan(psi) has unit length, is normal to the
+    fibre and psi = atan2(f.b, f.a) inverts it (the rule of the frame kernel)."""
+    rng = np.random.default_rng(1)
+    th, ph, psi = rng.uniform(0, np.pi, 500), rng.uniform(-np.pi, np.pi, 500), rng.uniform(-np.pi, np.pi, 500)
+    f = fan_dir(th, ph, psi)
+    n = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], -1)
+    np.testing.assert_allclose(np.linalg.norm(f, axis=1), 1.0, atol=1e-12)
+    np.testing.assert_allclose(np.sum(f * n, 1), 0.0, atol=1e-12)
+    a = np.stack([-np.sin(ph), np.cos(ph), np.zeros_like(ph)], -1)
+    b = np.stack([-np.cos(th) * np.cos(ph), -np.cos(th) * np.sin(ph), np.sin(th)], -1)
+    back = np.arctan2(np.sum(f * b, 1), np.sum(f * a, 1))
+    np.testing.assert_allclose(np.angle(np.exp(1j * (back - psi))), 0.0, atol=1e-10)
+
+
+def test_noise_sigma2_estimate():
+    """The b=0-spread estimate of sigma^2 (chi^2 median corrected) recovers
+    the noise variance of a homogeneous volume."""
+    import torch
+    from cudimot_amd import pipelines, synth
+    cfp = synth.model_acquisition("NODDI_Bingham")
+    rng = np.random.default_rng(3)
+    meas = 1000.0 + 40.0 * rng.normal(size=(20000, cfp.shape[0]))
+    s2 = pipelines.noise_sigma2(torch.as_tensor(meas), cfp)
+    assert abs(s2 / 1600.0 - 1.0) < 0.03, s2
+
+
+@pytest.mark.gpu
+def test_tensor_frame_matches_eigh():
+    import torch
+    from cudimot_amd import capi
+    rng = np.random.default_rng(0)
+    n = 2000
+    Q = np.linalg.qr(rng.normal(size=(n, 3, 3)))[0]
+    ev = -np.sort(-rng.uniform(0.2e-3, 3e-3, (n, 3)), 1)
+    D = np.einsum("nij,nj,nkj->nik", Q, ev, Q)
+    dti = np.stack([np.full(n, 1000.0), D[:, 0, 0], D[:, 0, 1], D[:, 0, 2], D[:, 1, 1], D[:, 1, 2], D[:, 2, 2]], 1)
+    out = torch.empty((n, 6), device="cuda")
+    capi.tensor_frame(dev(dti), out)
+    o = host(out)
+    w, V = np.linalg.eigh(np.asarray(dev(dti).cpu().numpy(), dtype=np.float64)[:, [1, 2, 3, 2, 4, 5, 3, 5, 6]]
+                         .reshape(n, 3, 3))
+    n1 = np.stack([np.sin(o[:, 0]) * np.cos(o[:, 1]), np.sin(o[:, 0]) * np.sin(o[:, 1]), np.cos(o[:, 0])], 1)
+    gap = np.minimum(ev[:, 0] - ev[:, 1], ev[:, 1] - ev[:, 2]) / ev[:, 0]
+    ok = gap > 1e-3  # eigenvectors of nearly equal eigenvalues are not defined
+    np.testing.assert_allclose(np.abs(np.sum(n1 * V[:, :, 2], 1))[ok], 1.0, atol=1e-5)
+    f = fan_dir(o[:, 0], o[:, 1], o[:, 2])
+    np.testing.assert_allclose(np.abs(np.sum(f * V[:, :, 1], 1))[ok], 1.0, atol=1e-4)
+    np.testing.assert_allclose(o[:, 3:], w[:, ::-1], rtol=1e-5, atol=1e-9)
+
+
+def _volume(n, seed, snr=25.0):
+    import torch
+    from cudimot_amd import capi, synth
+    cfp = synth.model_acquisition("NODDI_Bingham")
+    t = synth.truth_for("NODDI_Bingham", n, seed)
+    m = capi.Model("NODDI_Bingham")
+    s0 = torch.full((n, 1), 1000.0, device="cuda")
+    clean = torch.empty((n, cfp.shape[0]), device="cuda")
+    m.predicted_signal(dev(t.reshape(n, 7, 1)), dev(cfp), s0, clean)
+    c = host(clean)
+    rng = np.random.default_rng(seed + 5)
+    sig = 1000.0 / snr
+    meas = np.sqrt((c + sig * rng.normal(size=c.shape)) ** 2 + (sig * rng.normal(size=c.shape)) ** 2)
+    return t, cfp, s0, dev(meas)
+
+
+_NET = {}
+
+
+def _net():
+    if "net" not in _NET:
+        from cudimot_amd import learned
+        _NET["net"], _ = learned.train("NODDI_Bingham", n=262144, epochs=10, batch=1024, seed=17)
+    return _NET["net"]
+
+
+def _rmse(p, t, k):
+    return float(np.sqrt(np.mean((host(p)[:, k] - t[:, k]) ** 2)))
+
+
+@pytest.mark.gpu
+def test_learned_pipeline_matches_gridsearch_pipeline():
+    """Learned start + LM reaches the parameter error of the reference's
+    tensor fit + GridSearch + LM recipe on the same held-out voxels."""
+    from cudimot_amd import pipelines
+    t, cfp, s0, meas = _volume(4096, 901)
+    pl, cl, fl = pipelines.bingham_learned_lm(meas, cfp, s0, _net())
+    pg, cg = pipelines.bingham_gridsearch_lm(meas, cfp, s0)
+    for k in (0, 1):
+        assert _rmse(pl, t, k) <= 1.15 * _rmse(pg, t, k), (k, _rmse(pl, t, k), _rmse(pg, t, k))
+    for k in (2, 3):
+        assert _rmse(pl, t, k) <= 1.10 * _rmse(pg, t, k), (k, _rmse(pl, t, k), _rmse(pg, t, k))
+    assert np.median(host(cl)) <= 1.005 * np.median(host(cg))
+    assert host(fl).mean() < 0.05
+
+
+@pytest.mark.gpu
+def test_fallback_keeps_the_lower_residual():
+    """With a zero noise bound every voxel is refitted by GridSearch + LM and
+    the result per voxel is the lower of the two residuals."""
+    from cudimot_amd import pipelines
+    t, cfp, s0, meas = _volume(512, 902)
+    net = _net()
+    _, c_only, _ = pipelines.bingham_learned_lm(meas, cfp, s0, net, fallback=False)
+    _, c_gs = pipelines.bingham_gridsearch_lm(meas, cfp, s0)
+    p, c, fl = pipelines.bingham_learned_lm(meas, cfp, s0, net, factor=0.0)
+    assert host(fl).all()
+    np.testing.assert_allclose(host(c), np.minimum(host(c_only), host(c_gs)), rtol=1e-6)
