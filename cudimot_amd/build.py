@@ -65,7 +65,100 @@ def _compile(src, obj, deps, extra=()):
     return obj
 
 
-def build(models=None, extra_dirs=(), force=False, jobs=None):
+# Batch units: sets of plug-ins compiled into one persistent kernel per
+# method (cdm_batch_levmar / cdm_batch_mcmc).  The default set is the batched
+# workload of tensor, two-fan Bingham-NODDI and three-stick voxels.
+DEFAULT_BATCHES = {"tensor_bingham2_sticks3": ["DiffusionTensor", "NODDI_Bingham_2Fans", "Ball_3_Sticks"]}
+
+PLUGIN_MACROS = ["MODEL_H_INCLUDED", "NPARAMS", "NCFP", "NFIXP", "CDM_LM_JACOBIAN_DOUBLE", "CDM_VOXEL_CACHE",
+                 "CDM_VOXEL_CACHE_DEPENDS", "CDM_MEAS_CACHE", "CDM_NUMERICAL_MASK"]
+
+
+def batch_unit_source(name, model_dirs):
+    """Source of one batch unit: each plug-in in its own namespace with the
+    engine bodies, then the persistent kernels and the host-side tables."""
+    L = [1, 4, 16, 64]
+    out = ["// generated: batch unit %s (%s)" % (name, ", ".join(m for m, _ in model_dirs)),
+           '#include "abi/functions_gpu.h"', '#include "engine/philox.h"', '#include "engine/wave.h"',
+           '#include "host/model_ops.h"', "#include <algorithm>", "#include <climits>", "#include <cstring>", ""]
+    for k, (m, d) in enumerate(model_dirs):
+        out += ["#undef %s" % x for x in PLUGIN_MACROS]
+        out += ["namespace cdm_batch_%s_m%d {" % (name, k),
+                '#include "%s/modelparameters.h"' % d, '#include "%s/modelparameters.cc"' % d,
+                '#include "abi/macro_numerical.h"', '#include "%s/modelfunctions.h"' % d,
+                '#include "engine/engine.h"', '#include "engine/args.inl"',
+                "template <int L>",
+                "__device__ __noinline__ void batch_lm(const void* a, int marquardt, int blk) {",
+                "  cdm_levmar_body<L>(*static_cast<const cdm_lm_args*>(a), marquardt, blk);", "}",
+                "template <int L>",
+                "__device__ __noinline__ void batch_mcmc(const void* a, int blk) {",
+                "  cdm_mcmc_body<L, false>(*static_cast<const cdm_mcmc_args*>(a), blk);", "}",
+                "}  // namespace", ""]
+    ns = lambda k: "cdm_batch_%s_m%d" % (name, k)
+    sizes = ", ".join("sizeof(%s::cdm_lm_args), sizeof(%s::cdm_mcmc_args)" % (ns(k), ns(k))
+                      for k in range(len(model_dirs)))
+    out += ["namespace cdm_batch_%s {" % name,
+            "constexpr size_t kSizes[] = {%s};" % sizes,
+            "constexpr size_t arg_bytes() { size_t m = 0; for (size_t s : kSizes) m = s > m ? s : m; return m; }",
+            "struct DevItem {", "  int kase, first_tile, ntiles, pad;", "  alignas(16) unsigned char args[arg_bytes()];",
+            "};", "",
+            "template <int MODE>",
+            "__global__ void __launch_bounds__(256) batch_kernel(const DevItem* __restrict__ items, int nitems,",
+            "                                                     int total_tiles, int* queue, int marquardt) {",
+            "  __shared__ int s_tile;", "  for (;;) {", "    __syncthreads();",
+            "    if (threadIdx.x == 0) s_tile = atomicAdd(queue, 1);", "    __syncthreads();",
+            "    const int tile = s_tile;", "    if (tile >= total_tiles) break;", "    int it = 0;",
+            "    while (it + 1 < nitems && tile >= items[it + 1].first_tile) ++it;",
+            "    const DevItem* d = &items[it];", "    const int blk = tile - d->first_tile;", "    switch (d->kase) {"]
+    for k in range(len(model_dirs)):
+        for li, l in enumerate(L):
+            out.append("      case %d: if constexpr (MODE == 0) %s::batch_lm<%d>(d->args, marquardt, blk); "
+                       "else %s::batch_mcmc<%d>(d->args, blk); break;" % (4 * k + li, ns(k), l, ns(k), l))
+    out += ["      default: break;", "    }", "  }", "}", ""]
+    out += ["static int prepare(int mode, int nitems, const cdm::BatchLaunchItem* in, void* host_out, int* total_tiles,",
+            "                   size_t* lds_bytes) {",
+            "  DevItem* out = static_cast<DevItem*>(host_out);", "  int tiles = 0;", "  size_t lds = 0;",
+            "  for (int i = 0; i < nitems; ++i) {", "    const cdm::BatchLaunchItem& b = in[i];",
+            "    const int li = b.L == 1 ? 0 : b.L == 4 ? 1 : b.L == 16 ? 2 : b.L == 64 ? 3 : -1;",
+            "    if (li < 0 || b.model < 0 || b.model >= %d || b.g.nvox <= 0) return -1;" % len(model_dirs),
+            "    DevItem& d = out[i];", "    std::memset(&d, 0, sizeof(d));", "    d.kase = 4 * b.model + li;",
+            "    d.first_tile = tiles;", "    const int per = 256 / b.L;", "    d.ntiles = (b.g.nvox + per - 1) / per;",
+            "    tiles += d.ntiles;", "    size_t need = 0;", "    switch (b.model) {"]
+    for k in range(len(model_dirs)):
+        out += ["      case %d: {" % k,
+                "        if (b.g.cfp_ts > %s::cdm_cfp_max || b.g.fixp_ts > %s::cdm_fixp_max) return -1;" % (ns(k), ns(k)),
+                "        need = %s::op_lds(b.g);" % ns(k),
+                "        if (mode == cdm::kBatchLevMar) {",
+                "          const auto a = %s::op_lm_args(b.g, b.params, b.max_iters, b.c, b.iters, b.cost);" % ns(k),
+                "          std::memcpy(d.args, &a, sizeof(a));", "        } else {",
+                "          const auto a = %s::op_mcmc_args(b.g, b.mc, b.c);" % ns(k),
+                "          std::memcpy(d.args, &a, sizeof(a));", "        }", "      } break;"]
+    out += ["      default: return -1;", "    }", "    lds = std::max(lds, need);", "  }",
+            "  *total_tiles = tiles;", "  *lds_bytes = lds;", "  return 0;", "}", "",
+            "static hipError_t grid(int mode, int total_tiles, size_t lds, int* out) {",
+            "  auto kern = mode == cdm::kBatchLevMar ? batch_kernel<0> : batch_kernel<1>;",
+            "  int dev = 0, cus = 0, per_cu = 0;", "  hipError_t e = hipGetDevice(&dev);",
+            "  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);",
+            "  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, lds);",
+            "  if (e != hipSuccess) return e;",
+            "  *out = std::max(1, std::min(total_tiles, std::max(1, cus) * std::max(1, per_cu)));",
+            "  return hipSuccess;", "}", "",
+            "static hipError_t launch(int mode, const void* dev_items, int nitems, int total_tiles, int* queue,",
+            "                         size_t lds, int marquardt, int grid, hipStream_t s) {",
+            "  if (total_tiles <= 0) return hipSuccess;",
+            "  auto kern = mode == cdm::kBatchLevMar ? batch_kernel<0> : batch_kernel<1>;",
+            "  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, static_cast<const DevItem*>(dev_items), nitems,",
+            "                     total_tiles, queue, marquardt);",
+            "  return hipGetLastError();", "}", "",
+            "#ifndef __HIP_DEVICE_COMPILE__  // host tables (constant initialisers would be copied to the device)",
+            "static const char* const kNames[] = {%s};" % ", ".join('"%s"' % m for m, _ in model_dirs),
+            "extern const cdm::BatchOps ops = {\"%s\", %d, kNames, sizeof(DevItem), &prepare, &grid, &launch};" %
+            (name, len(model_dirs)),
+            "#endif", "}  // namespace", ""]
+    return "\n".join(out)
+
+
+def build(models=None, extra_dirs=(), force=False, jobs=None, batches=None):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     dirs = {}
@@ -84,11 +177,26 @@ def build(models=None, extra_dirs=(), force=False, jobs=None):
             with open(src, "w") as f:
                 f.write(text)
         jobs_list.append((src, os.path.join(BUILD, "unit_%s.o" % name), engine_deps + [d]))
+    batches = DEFAULT_BATCHES if batches is None else batches
+    batches = {b: ms for b, ms in batches.items() if all(m in dirs for m in ms)}
+    for bname, ms in sorted(batches.items()):
+        src = os.path.join(BUILD, "batch_%s.hip" % bname)
+        text = batch_unit_source(bname, [(m, dirs[m]) for m in ms])
+        if not os.path.exists(src) or open(src).read() != text:
+            with open(src, "w") as f:
+                f.write(text)
+        jobs_list.append((src, os.path.join(BUILD, "batch_%s.o" % bname), engine_deps + [dirs[m] for m in ms]))
     reg = os.path.join(BUILD, "registry.cpp")
     lines = ['#include "host/model_ops.h"', "#include <cstring>"]
     for name in sorted(dirs):
         lines.append("namespace cdm_model_%s { extern const cdm::ModelOps ops; }" % name)
+    for bname in sorted(batches):
+        lines.append("namespace cdm_batch_%s { extern const cdm::BatchOps ops; }" % bname)
     lines.append("namespace cdm {")
+    lines.append("static const BatchOps* const kBatches[] = {%s};" %
+                 ", ".join(["&cdm_batch_%s::ops" % b for b in sorted(batches)] + ["nullptr"]))
+    lines.append("int batch_count() { return %d; }" % len(batches))
+    lines.append("const BatchOps* batch_at(int i) { return kBatches[i]; }")
     lines.append("static const ModelOps* const kModels[] = {%s};" %
                  ", ".join("&cdm_model_%s::ops" % n for n in sorted(dirs)))
     lines.append("int model_count() { return (int)(sizeof(kModels) / sizeof(kModels[0])); }")
@@ -163,10 +271,13 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--models", default="", help="comma-separated shipped plug-ins (default: all)")
     ap.add_argument("--extra-model-dir", action="append", default=[], help="a plug-in directory to add")
+    ap.add_argument("--batch", action="append", default=[],
+                    help="a batch unit NAME=ModelA,ModelB,... (default: %s)" % DEFAULT_BATCHES)
     ap.add_argument("--force", action="store_true")
     a = ap.parse_args(argv)
     models = [m for m in a.models.split(",") if m] or None
-    print(build(models, a.extra_model_dir, a.force))
+    batches = dict((b.split("=", 1)[0], b.split("=", 1)[1].split(",")) for b in a.batch) if a.batch else None
+    print(build(models, a.extra_model_dir, a.force, batches=batches))
     print(build_opcount(models))
 
 

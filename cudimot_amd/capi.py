@@ -59,6 +59,9 @@ def lib():
             "cdm_bic_aic": (i, [vp, i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp]),
             "cdm_rician_clean": (i, [i, i, vp]),
             "cdm_tensor_frame": (i, [i, vp, i, i, vp, vp]),
+            "cdm_batch_units": (i, [cp, i]),
+            "cdm_batch_levmar": (i, [i, vp, i, i, vp]),
+            "cdm_batch_mcmc": (i, [i, vp, i, i, i, i, i, u64, i, i, vp]),
             "cdm_mlp_create": (i, [i, i, i, vp, i, u64, ctypes.POINTER(vp)]),
             "cdm_mlp_destroy": (None, [vp]),
             "cdm_mlp_info": (i, [vp, ip, ip, ip, ip, ctypes.POINTER(i64)]),
@@ -207,6 +210,52 @@ class Model:
         ns = samples.shape[2]
         check(lib().cdm_bic_aic(self.h, nvox, nmeas, ns, 1 if rician else 0, ptr(meas), ptr(samples),
                                 ptr(tau_samples), ptr(cfp), ptr(fixp), ptr(bic), ptr(aic), _stream(stream)))
+
+
+class BatchItem(ctypes.Structure):
+    """cdm_batch_item."""
+    _fields_ = [("model", ctypes.c_void_p), ("nvox", ctypes.c_int), ("nmeas", ctypes.c_int),
+                ("meas", ctypes.c_void_p), ("params", ctypes.c_void_p), ("cfp", ctypes.c_void_p),
+                ("fixp", ctypes.c_void_p), ("iters", ctypes.c_void_p), ("cost", ctypes.c_void_p),
+                ("samples", ctypes.c_void_p), ("tau_samples", ctypes.c_void_p), ("voxel_offset", ctypes.c_longlong)]
+
+
+def batch_units():
+    n = lib().cdm_batch_units(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    lib().cdm_batch_units(buf, n + 1)
+    out = {}
+    for unit in buf.value.decode().split(";"):
+        if unit:
+            name, models = unit.split(":", 1)
+            out[name] = models.split(",")
+    return out
+
+
+def _batch_items(items):
+    """items: dicts with model (Model), meas, params, cfp, fixp (optional),
+    iters, cost, samples (optional) and voxel_offset (optional)."""
+    arr = (BatchItem * len(items))()
+    for k, it in enumerate(items):
+        meas = it["meas"]
+        arr[k] = BatchItem(it["model"].h.value, meas.shape[0], meas.shape[1], ptr(meas), ptr(it["params"]),
+                           ptr(it["cfp"]), ptr(it.get("fixp")), ptr(it.get("iters")), ptr(it.get("cost")),
+                           ptr(it.get("samples")), None, int(it.get("voxel_offset", 0)))
+    return arr
+
+
+def batch_levmar(items, max_iters=200, marquardt=True, stream=None):
+    arr = _batch_items(items)
+    check(lib().cdm_batch_levmar(len(items), ctypes.cast(arr, ctypes.c_void_p), int(max_iters),
+                                 1 if marquardt else 0, _stream(stream)))
+
+
+def batch_mcmc(items, nburn=1000, njumps=1250, sampleevery=25, update_proposal=True, seed=8219, jumps_per_launch=0,
+               use_graph=False, stream=None):
+    arr = _batch_items(items)
+    check(lib().cdm_batch_mcmc(len(items), ctypes.cast(arr, ctypes.c_void_p), int(nburn), int(njumps),
+                               int(sampleevery), 1 if update_proposal else 0, 0, int(seed), int(jumps_per_launch),
+                               1 if use_graph else 0, _stream(stream)))
 
 
 def tensor_frame(dti_params, out, first=1, stream=None):

@@ -1,0 +1,73 @@
+// Kernel-argument builders of one model (included in the model namespace
+// after engine.h): host-side geometry, bounds/priors and launch options into
+// the kernels' argument structs, the dynamic LDS size and the grid.  Used by
+// the per-model launchers (launch.inl) and by the batched launch.
+
+static cdm_consts op_consts(const cdm::Consts& in) {
+  cdm_consts c;
+  for (int i = 0; i < cdm_np; ++i) {
+    c.bound_types[i] = in.bound_types[i];
+    c.bmin[i] = in.bmin[i];
+    c.bmax[i] = in.bmax[i];
+    c.prior_types[i] = in.prior_types[i];
+    c.pa[i] = in.pa[i];
+    c.pb[i] = in.pb[i];
+    c.fixed[i] = in.fixed[i];
+  }
+  return c;
+}
+
+static cdm_geom op_geom(const cdm::Geom& g) {
+  cdm_geom k;
+  k.nvox = g.nvox;
+  k.nmeas = g.nmeas;
+  k.cfp_ts = g.cfp_ts;
+  k.fixp_ts = g.fixp_ts;
+  k.meas = g.meas;
+  k.cfp = g.cfp;
+  k.fixp = g.fixp;
+  return k;
+}
+
+static size_t op_lds(const cdm::Geom& g) {
+  return (size_t)cdm_mc_offset(g.nmeas, g.cfp_ts) * sizeof(float) + (size_t)g.nmeas * cdm_mc * sizeof(CDM_F64);
+}
+
+static dim3 op_grid(int nvox, int L) {
+  const int per_block = cdm_block / L;
+  return dim3((unsigned)((nvox + per_block - 1) / per_block));
+}
+
+static cdm_lm_args op_lm_args(const cdm::Geom& g, float* params, int max_iters, const cdm::Consts& c, int* iters,
+                              float* cost) {
+  cdm_lm_args a;
+  a.g = op_geom(g);
+  a.params = params;
+  a.max_iters = max_iters;
+  a.c = op_consts(c);
+  a.iters_out = iters;
+  a.cost_out = cost;
+  return a;
+}
+
+static cdm_mcmc_args op_mcmc_args(const cdm::Geom& g, const cdm::McmcLaunch& m, const cdm::Consts& c) {
+  cdm_mcmc_args a;
+  a.g = op_geom(g);
+  a.params = m.params;
+  a.state = (cdm_chain*)m.state;
+  a.samples = m.samples;
+  a.tau_samples = m.tau_samples;
+  a.nsamples = m.nsamples;
+  a.sampleevery = m.sampleevery;
+  a.upe = m.upe;
+  a.recording = m.recording;
+  a.update_prop = m.update_prop;
+  a.begin = m.begin;
+  a.count = m.count;
+  a.phase_len = m.phase_len;
+  a.jump_offset = m.jump_offset;
+  a.seed = m.seed;
+  a.voxel_offset = m.voxel_offset;
+  a.c = op_consts(c);
+  return a;
+}

@@ -488,11 +488,16 @@ struct cdm_jacobian_setup {
   }
 };
 
+// The kernel bodies take the block index as an argument so that the batched
+// persistent kernel (one launch over several models' voxels) can run them on
+// tiles it takes from a work queue.  A body stages the CFP table (two block
+// barriers, reached by every thread), then returns early for threads past
+// the last voxel; it has no barrier after that point.
 template <int L>
-__global__ void __launch_bounds__(cdm_block) cdm_levmar_kernel(cdm_lm_args a, int marquardt) {
+__device__ __forceinline__ void cdm_levmar_body(const cdm_lm_args& a, int marquardt, int block) {
   extern __shared__ float s_cfp[];
   const CDM_F64* s_mc = cdm_stage_cfp(a.g, s_cfp);
-  const int v = blockIdx.x * (cdm_block / L) + threadIdx.x / L;
+  const int v = block * (cdm_block / L) + threadIdx.x / L;
   const int lane = threadIdx.x % L;
   if (v >= a.g.nvox) return;
   const float* mv = a.g.meas + (size_t)v * a.g.nmeas;
@@ -653,6 +658,11 @@ __global__ void __launch_bounds__(cdm_block) cdm_levmar_kernel(cdm_lm_args a, in
   }
 }
 
+template <int L>
+__global__ void __launch_bounds__(cdm_block) cdm_levmar_kernel(cdm_lm_args a, int marquardt) {
+  cdm_levmar_body<L>(a, marquardt, blockIdx.x);
+}
+
 // --------------------------------------------------------------------- MCMC
 // MCMC.cu:327-683: single-site Metropolis-within-Gibbs with adaptive
 // proposal widths.  One launch runs `count` jumps of one phase (burn-in or
@@ -761,10 +771,10 @@ __device__ __forceinline__ double cdm_prior(const cdm_consts& c, int i, TP* p, i
 }
 
 template <int L, bool RICIAN>
-__global__ void __launch_bounds__(cdm_block) cdm_mcmc_kernel(cdm_mcmc_args a) {
+__device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block) {
   extern __shared__ float s_cfp[];
   const CDM_F64* s_mc = cdm_stage_cfp(a.g, s_cfp);
-  const int v = blockIdx.x * (cdm_block / L) + threadIdx.x / L;
+  const int v = block * (cdm_block / L) + threadIdx.x / L;
   const int lane = threadIdx.x % L;
   if (v >= a.g.nvox) return;
   const float* mv = a.g.meas + (size_t)v * a.g.nmeas;
@@ -954,6 +964,11 @@ __global__ void __launch_bounds__(cdm_block) cdm_mcmc_kernel(cdm_mcmc_args a) {
       if (RICIAN && !a.recording && a.tau_samples) a.tau_samples[(size_t)v * a.nsamples] = (float)tau;
     }
   }
+}
+
+template <int L, bool RICIAN>
+__global__ void __launch_bounds__(cdm_block) cdm_mcmc_kernel(cdm_mcmc_args a) {
+  cdm_mcmc_body<L, RICIAN>(a, blockIdx.x);
 }
 
 // ------------------------------------------------- predicted signal, BIC/AIC

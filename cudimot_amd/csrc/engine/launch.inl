@@ -1,41 +1,7 @@
 // Host-side launchers of one model's kernels (included in the model
-// namespace after engine.h).  Grid: ceil(nvox / (256/L)) blocks of 256
-// threads; dynamic LDS holds the staged CFP table plus padding.
-
-static cdm_consts op_consts(const cdm::Consts& in) {
-  cdm_consts c;
-  for (int i = 0; i < cdm_np; ++i) {
-    c.bound_types[i] = in.bound_types[i];
-    c.bmin[i] = in.bmin[i];
-    c.bmax[i] = in.bmax[i];
-    c.prior_types[i] = in.prior_types[i];
-    c.pa[i] = in.pa[i];
-    c.pb[i] = in.pb[i];
-    c.fixed[i] = in.fixed[i];
-  }
-  return c;
-}
-
-static cdm_geom op_geom(const cdm::Geom& g) {
-  cdm_geom k;
-  k.nvox = g.nvox;
-  k.nmeas = g.nmeas;
-  k.cfp_ts = g.cfp_ts;
-  k.fixp_ts = g.fixp_ts;
-  k.meas = g.meas;
-  k.cfp = g.cfp;
-  k.fixp = g.fixp;
-  return k;
-}
-
-static size_t op_lds(const cdm::Geom& g) {
-  return (size_t)cdm_mc_offset(g.nmeas, g.cfp_ts) * sizeof(float) + (size_t)g.nmeas * cdm_mc * sizeof(CDM_F64);
-}
-
-static dim3 op_grid(int nvox, int L) {
-  const int per_block = cdm_block / L;
-  return dim3((unsigned)((nvox + per_block - 1) / per_block));
-}
+// namespace after engine.h and args.inl).  Grid: ceil(nvox / (256/L)) blocks
+// of 256 threads; dynamic LDS holds the staged CFP table and the measurement
+// cache.
 
 #define CDM_DISPATCH_L(L, CALL)            \
   switch (L) {                             \
@@ -71,13 +37,7 @@ static hipError_t op_levmar(int L, const cdm::Geom& g, float* params, int max_it
                             const cdm::Consts& c, int* iters, float* cost, hipStream_t s) {
   if (g.nvox <= 0) return hipSuccess;
   if (g.cfp_ts > cdm_cfp_max || g.fixp_ts > cdm_fixp_max) return hipErrorInvalidValue;
-  cdm_lm_args a;
-  a.g = op_geom(g);
-  a.params = params;
-  a.max_iters = max_iters;
-  a.c = op_consts(c);
-  a.iters_out = iters;
-  a.cost_out = cost;
+  const cdm_lm_args a = op_lm_args(g, params, max_iters, c, iters, cost);
   CDM_DISPATCH_L(L, hipLaunchKernelGGL(cdm_levmar_kernel<LG>, op_grid(g.nvox, LG), dim3(cdm_block), op_lds(g), s, a,
                                        marquardt));
   return hipGetLastError();
@@ -87,24 +47,7 @@ static hipError_t op_mcmc(int L, int rician, const cdm::Geom& g, const cdm::Mcmc
                           hipStream_t s) {
   if (g.nvox <= 0) return hipSuccess;
   if (g.cfp_ts > cdm_cfp_max || g.fixp_ts > cdm_fixp_max) return hipErrorInvalidValue;
-  cdm_mcmc_args a;
-  a.g = op_geom(g);
-  a.params = m.params;
-  a.state = (cdm_chain*)m.state;
-  a.samples = m.samples;
-  a.tau_samples = m.tau_samples;
-  a.nsamples = m.nsamples;
-  a.sampleevery = m.sampleevery;
-  a.upe = m.upe;
-  a.recording = m.recording;
-  a.update_prop = m.update_prop;
-  a.begin = m.begin;
-  a.count = m.count;
-  a.phase_len = m.phase_len;
-  a.jump_offset = m.jump_offset;
-  a.seed = m.seed;
-  a.voxel_offset = m.voxel_offset;
-  a.c = op_consts(c);
+  const cdm_mcmc_args a = op_mcmc_args(g, m, c);
   if (rician) {
     CDM_DISPATCH_L(L, hipLaunchKernelGGL((cdm_mcmc_kernel<LG, true>), op_grid(g.nvox, LG), dim3(cdm_block),
                                          op_lds(g), s, a));

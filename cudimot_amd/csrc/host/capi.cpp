@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -24,6 +25,7 @@ struct cdm_model {
   // autotuned choices for this model: (measurement count, method, lanes)
   struct Tuned {
     int nmeas, method, lanes;
+    double ms;  // measured time of the tuning run (0 if not given)
   };
   std::vector<Tuned> tuned;
   int mcmc_block = 0;
@@ -142,6 +144,7 @@ int read_tuning(cdm_model* m, const std::string& path, std::string* err) {
     std::istringstream ss(line);
     std::string model, method;
     int nmeas = 0, lanes = 0;
+    double ms = 0;
     if (!(ss >> model)) continue;
     if (!(ss >> nmeas >> method >> lanes) || nmeas <= 0 || !valid_group(lanes)) {
       *err = path + ":" + std::to_string(lineno) + ": expected '<model> <nmeas> <method> <lanes>' with lanes 1/4/16/64";
@@ -155,7 +158,8 @@ int read_tuning(cdm_model* m, const std::string& path, std::string* err) {
       *err = path + ":" + std::to_string(lineno) + ": unknown method '" + method + "'";
       return -1;
     }
-    got.push_back({nmeas, k, lanes});
+    if (!(ss >> ms)) ms = 0;
+    got.push_back({nmeas, k, lanes, ms});
   }
   m->tuned = got;
   return (int)got.size();
@@ -249,6 +253,30 @@ int ensure_state(cdm_model* m, size_t bytes) {
   if (e != hipSuccess) return hip_fail(e, "MCMC chain state");
   m->state_bytes = bytes;
   return 0;
+}
+
+// Records the launches that `issue` makes on a private capture stream into a
+// HIP graph and replays it once on `s` (a capture needs a stream other than
+// the null stream; the replay keeps the caller's stream order).
+template <class F>
+hipError_t run_as_graph(hipStream_t s, F issue) {
+  hipStream_t cs = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e != hipSuccess) return e;
+  e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    const hipError_t ei = issue(cs);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cs, &g);
+    e = ei != hipSuccess ? ei : ec;
+    hipGraphExec_t exec = nullptr;
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipGraphLaunch(exec, s);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (g) (void)hipGraphDestroy(g);
+  }
+  (void)hipStreamDestroy(cs);
+  return e;
 }
 
 }  // namespace
@@ -459,35 +487,17 @@ int cdm_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params
     ml.update_prop = phase == 0 ? 1 : (update_proposal ? 1 : 0);
     ml.phase_len = len;
     ml.jump_offset = phase == 0 ? 0u : (unsigned)nb;
-    hipGraph_t graph = nullptr;
-    const bool use_graph = m->mcmc_graph && len > blk;
-    if (use_graph) {
-      hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-      if (e != hipSuccess) return hip_fail(e, "MCMC graph capture");
-    }
-    for (int b = 0; b < len; b += blk) {
-      ml.begin = b;
-      ml.count = std::min(blk, len - b);
-      const hipError_t e = m->ops->mcmc(L, rician ? 1 : 0, g, ml, c, s);
-      if (e != hipSuccess) {
-        if (use_graph) {
-          hipGraph_t dummy;
-          (void)hipStreamEndCapture(s, &dummy);
-          if (dummy) (void)hipGraphDestroy(dummy);
-        }
-        return hip_fail(e, "MCMC launch");
+    auto issue = [&](hipStream_t q) -> hipError_t {
+      for (int b = 0; b < len; b += blk) {
+        ml.begin = b;
+        ml.count = std::min(blk, len - b);
+        const hipError_t e = m->ops->mcmc(L, rician ? 1 : 0, g, ml, c, q);
+        if (e != hipSuccess) return e;
       }
-    }
-    if (use_graph) {
-      hipError_t e = hipStreamEndCapture(s, &graph);
-      if (e != hipSuccess) return hip_fail(e, "MCMC graph capture");
-      hipGraphExec_t exec = nullptr;
-      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-      if (e == hipSuccess) e = hipGraphLaunch(exec, s);
-      if (exec) (void)hipGraphExecDestroy(exec);
-      (void)hipGraphDestroy(graph);
-      if (e != hipSuccess) return hip_fail(e, "MCMC graph launch");
-    }
+      return hipSuccess;
+    };
+    const hipError_t e = (m->mcmc_graph && len > blk) ? run_as_graph(s, issue) : issue(s);
+    if (e != hipSuccess) return hip_fail(e, "MCMC launch");
   }
   return 0;
 }
@@ -535,6 +545,239 @@ int cdm_rician_clean(int nvox, int nmeas, float* meas) {
     }
   }
   return 0;
+}
+
+int cdm_batch_units(char* buf, int buflen) {
+  std::string s;
+  for (int i = 0; i < cdm::batch_count(); ++i) {
+    const cdm::BatchOps* b = cdm::batch_at(i);
+    if (i) s += ";";
+    s += b->name;
+    s += ":";
+    for (int k = 0; k < b->nmodels; ++k) {
+      if (k) s += ",";
+      s += b->models[k];
+    }
+  }
+  if (buf && buflen > 0) {
+    std::strncpy(buf, s.c_str(), (size_t)buflen - 1);
+    buf[buflen - 1] = 0;
+  }
+  return (int)s.size();
+}
+
+}  // extern "C"
+
+namespace {
+
+// Batch unit holding every item's model, and each item's model index in it.
+const cdm::BatchOps* find_batch(int nitems, const cdm_batch_item* items, std::vector<int>* index) {
+  for (int u = 0; u < cdm::batch_count(); ++u) {
+    const cdm::BatchOps* b = cdm::batch_at(u);
+    std::vector<int> idx;
+    for (int i = 0; i < nitems; ++i) {
+      int k = -1;
+      for (int j = 0; j < b->nmodels; ++j)
+        if (items[i].model && std::strcmp(b->models[j], items[i].model->ops->name) == 0) k = j;
+      if (k < 0) break;
+      idx.push_back(k);
+    }
+    if ((int)idx.size() == nitems) {
+      *index = idx;
+      return b;
+    }
+  }
+  return nullptr;
+}
+
+int check_batch(int nitems, const cdm_batch_item* items, bool mcmc, const cdm::BatchOps** unit,
+                std::vector<int>* index, long long* total_vox) {
+  if (nitems <= 0 || !items) return fail("a batch needs at least one item");
+  *total_vox = 0;
+  for (int i = 0; i < nitems; ++i) {
+    const cdm_batch_item& it = items[i];
+    int r = check_geom(it.model, it.nvox, it.nmeas, it.meas, it.cfp, it.fixp, true);
+    if (r) return fail("batch item " + std::to_string(i) + ": " + g_err);
+    if (it.nvox <= 0) return fail("batch item " + std::to_string(i) + ": no voxels");
+    if (!it.params) return fail("batch item " + std::to_string(i) + ": params pointer is null");
+    if (mcmc && !it.samples) return fail("batch item " + std::to_string(i) + ": samples pointer is null");
+    for (int j = 0; j < i; ++j)
+      if (items[j].model == it.model)
+        return fail("batch items " + std::to_string(j) + " and " + std::to_string(i) +
+                    " share a model handle (each item needs its own handle: it owns the chain state)");
+    *total_vox += it.nvox;
+  }
+  *unit = find_batch(nitems, items, index);
+  if (!*unit) {
+    std::string names;
+    for (int i = 0; i < nitems; ++i) names += (i ? "," : "") + std::string(items[i].model->ops->name);
+    return fail("no batch unit compiled for the models " + names +
+                " (build with python -m cudimot_amd.build --batch NAME=" + names + ")");
+  }
+  return 0;
+}
+
+// Items ordered by estimated work per voxel (heaviest first), so the queue
+// hands out the long tiles early and the short ones fill the tail.  The
+// estimate is the tuning table's time for the model and method when present,
+// else measurements x parameters.
+std::vector<int> batch_order(int nitems, const cdm_batch_item* items, int method) {
+  std::vector<double> w(nitems);
+  for (int i = 0; i < nitems; ++i) {
+    const cdm_model* m = items[i].model;
+    double t = 0;
+    for (const auto& e : m->tuned)
+      if (e.method == method && e.nmeas > 0) t = std::max(t, e.ms * items[i].nmeas / e.nmeas);
+    w[i] = t > 0 ? t : (double)items[i].nmeas * m->ops->nparams;
+  }
+  std::vector<int> order(nitems);
+  for (int i = 0; i < nitems; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
+  return order;
+}
+
+struct BatchBuffers {
+  void* items = nullptr;
+  int* queues = nullptr;
+  ~BatchBuffers() {
+    if (items) (void)hipFree(items);
+    if (queues) (void)hipFree(queues);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int cdm_batch_levmar(int nitems, const cdm_batch_item* items, int max_iters, int marquardt, void* stream) {
+  const cdm::BatchOps* unit = nullptr;
+  std::vector<int> index;
+  long long total = 0;
+  int r = check_batch(nitems, items, false, &unit, &index, &total);
+  if (r) return r;
+  hipStream_t s = (hipStream_t)stream;
+  const std::vector<int> order = batch_order(nitems, items, cdm::kLevMar);
+  std::vector<cdm::BatchLaunchItem> li(nitems);
+  for (int k = 0; k < nitems; ++k) {
+    const cdm_batch_item& it = items[order[k]];
+    cdm::BatchLaunchItem& b = li[k];
+    b.model = index[order[k]];
+    b.L = group_for(it.model, cdm::kLevMar, it.nmeas, (int)std::min<long long>(total, INT_MAX));
+    b.g = make_geom(it.model, it.nvox, it.nmeas, it.meas, it.cfp, it.fixp);
+    b.c = make_consts(it.model);
+    b.params = it.params;
+    b.max_iters = max_iters;
+    b.iters = it.iters;
+    b.cost = it.cost;
+  }
+  std::vector<unsigned char> host(unit->item_bytes * nitems);
+  int tiles = 0;
+  size_t lds = 0;
+  if (unit->prepare(cdm::kBatchLevMar, nitems, li.data(), host.data(), &tiles, &lds))
+    return fail("batch LM: an item does not fit the engine limits");
+  BatchBuffers buf;
+  hipError_t e = hipMalloc(&buf.items, host.size());
+  if (e == hipSuccess) e = hipMalloc((void**)&buf.queues, sizeof(int));
+  if (e == hipSuccess) e = hipMemcpyAsync(buf.items, host.data(), host.size(), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemsetAsync(buf.queues, 0, sizeof(int), s);
+  int grid = 0;
+  if (e == hipSuccess) e = unit->grid(cdm::kBatchLevMar, tiles, lds, &grid);
+  if (e == hipSuccess)
+    e = unit->launch(cdm::kBatchLevMar, buf.items, nitems, tiles, buf.queues, lds, marquardt ? 1 : 0, grid, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e == hipSuccess ? 0 : hip_fail(e, "batch LM");
+}
+
+int cdm_batch_mcmc(int nitems, const cdm_batch_item* items, int nburn, int njumps, int sampleevery,
+                   int update_proposal, int rician, unsigned long long seed, int jumps_per_launch, int use_graph,
+                   void* stream) {
+  const cdm::BatchOps* unit = nullptr;
+  std::vector<int> index;
+  long long total = 0;
+  int r = check_batch(nitems, items, true, &unit, &index, &total);
+  if (r) return r;
+  if (rician) return fail("batched MCMC is compiled with the Gaussian likelihood only (run --rician per model)");
+  if (sampleevery <= 0) return fail("--sampleevery must be > 0");
+  if (njumps < sampleevery || njumps % sampleevery != 0)
+    return fail("--nj must be a positive multiple of --se");
+  if (jumps_per_launch < 0) return fail("jumps per launch must be >= 0");
+  const int nb = nburn < 1 ? 1 : nburn;  // MCMC.cu:702-703
+  const int nsamples = njumps / sampleevery;
+  hipStream_t s = (hipStream_t)stream;
+  for (int i = 0; i < nitems; ++i)
+    if ((r = ensure_state(items[i].model, items[i].model->ops->chain_bytes * (size_t)items[i].nvox))) return r;
+  const std::vector<int> order = batch_order(nitems, items, cdm::kMcmc);
+  // one prepared item set per launch: phases x jump blocks
+  struct Launch {
+    int phase, begin, count;
+  };
+  std::vector<Launch> launches;
+  for (int phase = 0; phase < 2; ++phase) {
+    const int len = phase == 0 ? nb : njumps;
+    const int blk = jumps_per_launch > 0 ? jumps_per_launch : len;
+    for (int b = 0; b < len; b += blk) launches.push_back({phase, b, std::min(blk, len - b)});
+  }
+  const size_t per = unit->item_bytes * nitems;
+  std::vector<unsigned char> host(per * launches.size());
+  std::vector<cdm::BatchLaunchItem> li(nitems);
+  int tiles = 0;
+  size_t lds = 0;
+  for (size_t q = 0; q < launches.size(); ++q) {
+    const Launch& L = launches[q];
+    for (int k = 0; k < nitems; ++k) {
+      const cdm_batch_item& it = items[order[k]];
+      cdm::BatchLaunchItem& b = li[k];
+      b.model = index[order[k]];
+      b.L = group_for(it.model, cdm::kMcmc, it.nmeas, (int)std::min<long long>(total, INT_MAX));
+      b.g = make_geom(it.model, it.nvox, it.nmeas, it.meas, it.cfp, it.fixp);
+      b.c = make_consts(it.model);
+      b.params = it.params;
+      cdm::McmcLaunch& ml = b.mc;
+      ml.params = it.params;
+      ml.state = it.model->d_state;
+      ml.samples = it.samples;
+      ml.tau_samples = it.tau_samples;
+      ml.nsamples = nsamples;
+      ml.sampleevery = sampleevery;
+      ml.upe = sampleevery;
+      ml.seed = seed;
+      ml.voxel_offset = it.voxel_offset;
+      ml.recording = L.phase;
+      ml.update_prop = L.phase == 0 ? 1 : (update_proposal ? 1 : 0);
+      ml.phase_len = L.phase == 0 ? nb : njumps;
+      ml.jump_offset = L.phase == 0 ? 0u : (unsigned)nb;
+      ml.begin = L.begin;
+      ml.count = L.count;
+    }
+    if (unit->prepare(cdm::kBatchMcmc, nitems, li.data(), host.data() + q * per, &tiles, &lds))
+      return fail("batch MCMC: an item does not fit the engine limits");
+  }
+  BatchBuffers buf;
+  hipError_t e = hipMalloc(&buf.items, host.size());
+  if (e == hipSuccess) e = hipMalloc((void**)&buf.queues, sizeof(int) * launches.size());
+  if (e == hipSuccess) e = hipMemcpyAsync(buf.items, host.data(), host.size(), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemsetAsync(buf.queues, 0, sizeof(int) * launches.size(), s);
+  if (e != hipSuccess) return hip_fail(e, "batch MCMC set-up");
+  int grid = 0;
+  if ((e = unit->grid(cdm::kBatchMcmc, tiles, lds, &grid)) != hipSuccess) return hip_fail(e, "batch MCMC set-up");
+  size_t q = 0;
+  for (int phase = 0; phase < 2 && e == hipSuccess; ++phase) {
+    size_t q_end = q;
+    while (q_end < launches.size() && launches[q_end].phase == phase) ++q_end;
+    const size_t q0 = q;
+    auto issue = [&](hipStream_t st) -> hipError_t {
+      for (size_t k = q0; k < q_end; ++k) {
+        const hipError_t el = unit->launch(cdm::kBatchMcmc, (unsigned char*)buf.items + k * per, nitems, tiles,
+                                           buf.queues + k, lds, 0, grid, st);
+        if (el != hipSuccess) return el;
+      }
+      return hipSuccess;
+    };
+    e = (use_graph && q_end - q0 > 1) ? run_as_graph(s, issue) : issue(s);
+    q = q_end;
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e == hipSuccess ? 0 : hip_fail(e, "batch MCMC");
 }
 
 int cdm_sync(void* stream) {

@@ -69,6 +69,48 @@ const ModelOps* find_model(const char* name);
 int model_count();
 const ModelOps* model_at(int i);
 
+// ---------------------------------------------------------------- batches
+// A batch unit compiles several plug-ins into one persistent kernel per
+// method: workgroups take voxel tiles (one block's worth of one item) from a
+// work queue until every item's tiles are done.
+enum BatchMode { kBatchLevMar = 0, kBatchMcmc = 1 };
+
+struct BatchLaunchItem {
+  int model;  // index in the unit's model list
+  int L;      // voxel-group size
+  Geom g;
+  Consts c;
+  // Levenberg-Marquardt
+  float* params;
+  int max_iters;
+  int* iters;
+  float* cost;
+  // MCMC
+  McmcLaunch mc;
+};
+
+struct BatchOps {
+  const char* name;
+  int nmodels;
+  const char* const* models;
+  size_t item_bytes;  // device bytes of one prepared item
+  // Host-side preparation of nitems device items (into host_out, item_bytes
+  // each, in the given order); returns the total number of tiles and the
+  // dynamic LDS bytes a workgroup needs.  Non-zero on an invalid item.
+  int (*prepare)(int mode, int nitems, const BatchLaunchItem* items, void* host_out, int* total_tiles,
+                 size_t* lds_bytes);
+  // Workgroups of the persistent launch: resident blocks per CU x CUs,
+  // capped by the tiles (queried before any stream capture).
+  hipError_t (*grid)(int mode, int total_tiles, size_t lds_bytes, int* grid);
+  // One persistent launch over prepared items (dev_items) with a zeroed
+  // work-queue counter.
+  hipError_t (*launch)(int mode, const void* dev_items, int nitems, int total_tiles, int* queue, size_t lds_bytes,
+                       int marquardt, int grid, hipStream_t s);
+};
+
+int batch_count();
+const BatchOps* batch_at(int i);
+
 // Error reporting shared by every C-ABI source file: stores the message for
 // cdm_last_error() on this thread and returns the status (-1 argument error,
 // -2 HIP error).

@@ -191,6 +191,45 @@ int cdm_mlp_train(cdm_mlp* m, int n, int nmeas, const float* clean, const float*
                   const float* params_true, float snr_lo, float snr_hi, int epochs, int batch, float lr,
                   unsigned long long seed, float* epoch_loss, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Batched persistent launch.  No counterpart in the reference, which runs one
+ * binary per model over its voxel sub-parts in turn (cudimot.cc:81-120).
+ * Volumes of several models (each item: its own model handle, measurement
+ * count and buffers, with the layouts above) are fitted by ONE launch per
+ * method: workgroups take voxel tiles from a work queue, heaviest items
+ * first, until all are done.  The models must be compiled together into a
+ * batch unit (python -m cudimot_amd.build --batch NAME=A,B,...); each item
+ * uses its model handle's bounds, priors, fixed parameters and tuned
+ * voxel-group size.  The calls return when the work has finished. */
+typedef struct cdm_batch_item {
+  cdm_model* model; /* a model handle used by no other item of the batch */
+  int nvox, nmeas;
+  const float* meas;
+  float* params;
+  const float* cfp;
+  const float* fixp;
+  int* iters;   /* LM outputs (nullable) */
+  float* cost;
+  float* samples;     /* MCMC outputs: [nvox][nparams][njumps / sampleevery] */
+  float* tau_samples; /* unused (Gaussian likelihood) */
+  long long voxel_offset; /* MCMC random-stream offset */
+} cdm_batch_item;
+
+/* "name:ModelA,ModelB;name2:..." of the compiled batch units.  Returns the
+ * full length needed. */
+int cdm_batch_units(char* buf, int buflen);
+
+/* Levenberg_Marquardt<T>::run over every item in one launch. */
+int cdm_batch_levmar(int nitems, const cdm_batch_item* items, int max_iters, int marquardt, void* stream);
+
+/* MCMC<T>::run over every item: one launch per jump block (jumps_per_launch
+ * jumps, 0: a whole phase), each over all items; use_graph != 0 records a
+ * phase's launches into a HIP graph.  Draws are those of cdm_mcmc with the
+ * same seed and voxel offsets.  Gaussian likelihood only (rician must be 0). */
+int cdm_batch_mcmc(int nitems, const cdm_batch_item* items, int nburn, int njumps, int sampleevery,
+                   int update_proposal, int rician, unsigned long long seed, int jumps_per_launch, int use_graph,
+                   void* stream);
+
 /* Orientation initialisation from a tensor fit (the reference pipeline's
  * dtifit V1/V2 -> cart2spherical -> initialise_Psi steps: utils/Run_dtifit.sh:36-46,
  * utils/cart2spherical.cc:17-69, utils/initialise_Psi.cc:18-106).  dti
