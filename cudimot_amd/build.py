@@ -54,6 +54,26 @@ def _deps_hash(paths):
     return h.hexdigest()
 
 
+def _plugin_deps(d, seen=None):
+    """A plug-in directory plus the directories its headers include from
+    (e.g. a sibling plug-in's helper header)."""
+    import re
+    seen = set() if seen is None else seen
+    d = os.path.abspath(d)
+    if d in seen or not os.path.isdir(d):
+        return seen
+    seen.add(d)
+    for f in os.listdir(d):
+        fp = os.path.join(d, f)
+        if not os.path.isfile(fp):
+            continue
+        for inc in re.findall(r'#include\s+"([^"]+)"', open(fp, errors="replace").read()):
+            tgt = os.path.dirname(os.path.abspath(os.path.join(d, inc)))
+            if tgt != d:
+                _plugin_deps(tgt, seen)
+    return seen
+
+
 def _compile(src, obj, deps, extra=()):
     stamp = obj + ".sha"
     key = _deps_hash([src] + list(deps)) + ARCH + " ".join(extra)
@@ -176,7 +196,7 @@ def build(models=None, extra_dirs=(), force=False, jobs=None, batches=None):
         if not os.path.exists(src) or open(src).read() != text:
             with open(src, "w") as f:
                 f.write(text)
-        jobs_list.append((src, os.path.join(BUILD, "unit_%s.o" % name), engine_deps + [d]))
+        jobs_list.append((src, os.path.join(BUILD, "unit_%s.o" % name), engine_deps + sorted(_plugin_deps(d))))
     batches = DEFAULT_BATCHES if batches is None else batches
     batches = {b: ms for b, ms in batches.items() if all(m in dirs for m in ms)}
     for bname, ms in sorted(batches.items()):
@@ -185,7 +205,10 @@ def build(models=None, extra_dirs=(), force=False, jobs=None, batches=None):
         if not os.path.exists(src) or open(src).read() != text:
             with open(src, "w") as f:
                 f.write(text)
-        jobs_list.append((src, os.path.join(BUILD, "batch_%s.o" % bname), engine_deps + [dirs[m] for m in ms]))
+        bdeps = set()
+        for m in ms:
+            _plugin_deps(dirs[m], bdeps)
+        jobs_list.append((src, os.path.join(BUILD, "batch_%s.o" % bname), engine_deps + sorted(bdeps)))
     reg = os.path.join(BUILD, "registry.cpp")
     lines = ['#include "host/model_ops.h"', "#include <cstring>"]
     for name in sorted(dirs):
@@ -247,7 +270,7 @@ def build_opcount(models=None):
         if not os.path.exists(src) or open(src).read() != text:
             with open(src, "w") as f:
                 f.write(text)
-        jobs_list.append((src, os.path.join(BUILD, "opc_%s.o" % name), [d, os.path.join(CSRC, "tools"),
+        jobs_list.append((src, os.path.join(BUILD, "opc_%s.o" % name), sorted(_plugin_deps(d)) + [os.path.join(CSRC, "tools"),
                                                                           os.path.join(CSRC, "abi")]))
 
     def comp(src, obj, deps):

@@ -3,9 +3,10 @@
 // hyp_Sapprox_doublePrecision.h:8-131, eigenValues.h:12-51):
 //  * the confluent hypergeometric function of a 3x3 matrix argument,
 //    1F1(1/2; 3/2; X), by the second-order saddle-point approximation of
-//    Kume & Wood (Biometrika 92, 2005), always in double;
-//  * eigenvalues of a symmetric 3x3 matrix by Smith's trigonometric method
-//    (Comm. ACM 4, 1961);
+//    Kume & Wood (Biometrika 92, 2005), always in double; for the
+//    per-measurement argument through the characteristic polynomial instead
+//    of the reference's eigenvalues by Smith's method (the same
+//    approximation; eigenValues.h:12-51);
 //  * hindered diffusivities from central differences (delta 5e-5) of
 //    1F1(kappa, beta, 0), the parallel one in kappa and the first
 //    perpendicular one in beta, the second from the trace.
@@ -37,50 +38,76 @@ __device__ inline CDM_F64 bingham_saddle(CDM_F64 l1, CDM_F64 l2, CDM_F64 l3) {
 }
 
 // 1F1(1/2; 3/2; diag(x1, x2, x3)), normalised so that 1F1(0) = 1.
-//   c1 = sqrt(2 pi / K2) * pi^0... : c1 = sqrt(2/K2) pi prod_i (l_i - t)^-1/2 e^-t
+//   c1 = sqrt(2/K2) pi prod_i (l_i - t)^-1/2 e^-t
 //   c3 = c1 exp(K4 / (8 K2^2) - 5 K3^2 / (24 K2^3)),   result = c3 / (4 pi)
 // with K_j = (j-1)!/2 sum_i (l_i - t)^-j.
 __device__ inline CDM_F64 bingham_hyp(CDM_F64 x1, CDM_F64 x2, CDM_F64 x3) {
   if (x1 == 0.0 && x2 == 0.0 && x3 == 0.0) return 1.0;
-  const CDM_F64 pi = 3.14159265358979323846;
   const CDM_F64 l1 = -x1, l2 = -x2, l3 = -x3;
   const CDM_F64 t = bingham_saddle(l1, l2, l3);
   const CDM_F64 a = 1.0 / (l1 - t), b = 1.0 / (l2 - t), c = 1.0 / (l3 - t);
   const CDM_F64 K2 = 0.5 * (a * a + b * b + c * c);
   const CDM_F64 K3 = a * a * a + b * b * b + c * c * c;
   const CDM_F64 K4 = 3.0 * (a * a * a * a + b * b * b * b + c * c * c * c);
-  const CDM_F64 R = sqrt(a * b * c);
   const CDM_F64 T2 = K4 / (8.0 * K2 * K2) - 5.0 * K3 * K3 / (24.0 * K2 * K2 * K2);
-  const CDM_F64 c1 = sqrt(2.0 / K2) * pi * R * exp(-t);
-  return c1 * exp(T2) / (4.0 * pi);
+  return 0.25 * sqrt(2.0 * a * b * c / K2) * exp(T2 - t);
 }
 
-// Eigenvalues of a symmetric 3x3 matrix [a00 a01 a02; . a11 a12; . . a22],
-// in descending order (Smith's method), in double: the intra-axonal
-// normaliser feeds central differences in kappa and beta, whose steps are
-// far below single-precision resolution of the eigenvalues.
-__device__ inline void bingham_eig3(CDM_F64 a00, CDM_F64 a01, CDM_F64 a02, CDM_F64 a11, CDM_F64 a12, CDM_F64 a22,
-                                    CDM_F64* e) {
-  const CDM_F64 off = a01 * a01 + a02 * a02 + a12 * a12;
-  if (off == 0.0) {
-    e[0] = a00;
-    e[1] = a11;
-    e[2] = a22;
-    return;
+// The same approximation for X = diag(0, beta, kappa) - r g g^T (r >= 0, g a
+// unit vector), the intra-axonal argument of one measurement, without an
+// eigen-decomposition.  With l_i the eigenvalues of L = -X = diag(0, -beta,
+// -kappa) + r g g^T, everything the approximation needs is a symmetric
+// function of the l_i, so it follows from the coefficients of
+// E(s) = det(L - s I) = s3 - s2 s + s1 s^2 - s^3 (s1, s2, s3 the elementary
+// symmetric functions, by the matrix determinant lemma):
+//   * the smallest eigenvalue: E is convex and decreasing left of it, and
+//     l_min >= min(0, -beta, -kappa) (interlacing under the rank-one update),
+//     so Newton's method from that bound increases monotonically to it;
+//   * the saddle point: with y_i = 1 / (l_i - s), the elementary symmetric
+//     functions of the y_i are e1 = E2/E, e2 = E1/E, e3 = 1/E (E1 = s1 - 3s,
+//     E2 = s2 - 2 s1 s + 3 s^2), K'(s) = e1 / 2 and K''(s) = (e1^2 - 2 e2) / 2;
+//   * the power sums at t by Newton's identities give K2, K3, K4, and
+//     prod_i y_i = e3.
+__device__ inline CDM_F64 bingham_hyp_rank1(CDM_F64 beta, CDM_F64 kappa, CDM_F64 r, CDM_F64 gx, CDM_F64 gy,
+                                           CDM_F64 gn) {
+  const CDM_F64 d2 = -beta, d3 = -kappa;
+  const CDM_F64 w1 = gx * gx, w2 = gy * gy, w3 = gn * gn;
+  const CDM_F64 s1 = d2 + d3 + r * (w1 + w2 + w3);
+  const CDM_F64 s2 = d2 * d3 + r * (w1 * (d2 + d3) + w2 * d3 + w3 * d2);
+  const CDM_F64 s3 = r * w1 * d2 * d3;
+  // smallest eigenvalue of L
+  CDM_F64 s = fmin(d2, d3);
+  if (s > 0.0) s = 0.0;
+  for (int it = 0; it < 60; ++it) {
+    const CDM_F64 E = s3 - s * (s2 - s * (s1 - s));
+    if (!(E > 0.0)) break;
+    const CDM_F64 dE = -s2 + s * (2.0 * s1 - 3.0 * s);
+    if (!(dE < 0.0)) break;
+    const CDM_F64 sn = s - E / dE;
+    if (!(sn > s)) break;
+    s = sn;
   }
-  const CDM_F64 q = (a00 + a11 + a22) / 3.0;
-  const CDM_F64 d0 = a00 - q, d1 = a11 - q, d2 = a22 - q;
-  const CDM_F64 p = sqrt((d0 * d0 + d1 * d1 + d2 * d2 + 2.0 * off) / 6.0);
-  const CDM_F64 ip = 1.0 / p;
-  const CDM_F64 b00 = d0 * ip, b11 = d1 * ip, b22 = d2 * ip, b01 = a01 * ip, b02 = a02 * ip, b12 = a12 * ip;
-  const CDM_F64 r = (b00 * (b11 * b22 - b12 * b12) - b01 * (b01 * b22 - b02 * b12) + b02 * (b01 * b12 - b02 * b11)) / 2.0;
-  CDM_F64 phi;
-  if (r <= -1.0) phi = 3.14159265358979323846 / 3.0;
-  else if (r >= 1.0) phi = 0.0;
-  else phi = acos(r) / 3.0;
-  e[0] = q + 2.0 * p * cos(phi);
-  e[2] = q + 2.0 * p * cos(phi + 2.0 * 3.14159265358979323846 / 3.0);
-  e[1] = 3.0 * q - e[0] - e[2];
+  // saddle point t < l_min from l_min - 1/2 (K'(l_min - 1/2) >= 1)
+  CDM_F64 t = s - 0.5, E = 0, e1 = 0, e2 = 0;
+  for (int it = 0; it < 60; ++it) {
+    E = s3 - t * (s2 - t * (s1 - t));
+    const CDM_F64 iE = 1.0 / E;
+    e1 = (s2 - t * (2.0 * s1 - 3.0 * t)) * iE;
+    e2 = (s1 - 3.0 * t) * iE;
+    const CDM_F64 tn = t - (0.5 * e1 - 1.0) / (0.5 * (e1 * e1 - 2.0 * e2));
+    if (!(tn < t)) break;  // monotone from the right: stop when no progress
+    t = tn;
+  }
+  E = s3 - t * (s2 - t * (s1 - t));
+  const CDM_F64 e3 = 1.0 / E;
+  e1 = (s2 - t * (2.0 * s1 - 3.0 * t)) * e3;
+  e2 = (s1 - 3.0 * t) * e3;
+  const CDM_F64 p2 = e1 * e1 - 2.0 * e2;
+  const CDM_F64 p3 = e1 * p2 - e2 * e1 + 3.0 * e3;
+  const CDM_F64 p4 = e1 * p3 - e2 * p2 + e3 * e1;
+  const CDM_F64 K2 = 0.5 * p2, K3 = p3, K4 = 3.0 * p4;
+  const CDM_F64 T2 = K4 / (8.0 * K2 * K2) - 5.0 * K3 * K3 / (24.0 * K2 * K2 * K2);
+  return 0.25 * sqrt(2.0 * e3 / K2) * exp(T2 - t);
 }
 
 // Per-fan quantities that depend only on (fintra, kappa, beta): the
@@ -120,17 +147,16 @@ MACRO void bingham_fan_signal_parts(T kappa, T beta, T th, T ph, T psi, const bi
   const T b = CFP[3];
   // hindered compartment
   const T c2 = gn * gn, s2 = (T)1 - c2;
-  const T az = tatan2(gy, gx);
-  const T caz = cos_gpu(az);
-  const T caz2 = caz * caz, saz2 = (T)1 - caz2;
+  // cos^2 of the azimuth atan2(gy, gx) in the plane normal to the fibre
+  const T gxy = gx * gx + gy * gy;
+  const T caz2 = gxy > (T)0 ? gx * gx / gxy : (T)1, saz2 = (T)1 - caz2;
   extra = exp_gpu(-b * ((T)f.dpar * c2 + s2 * ((T)f.dperp1 * saz2 + (T)f.dperp2 * caz2)));
   // intra-axonal compartment: 1F1 of diag(0, beta, kappa) - b Dpar g_local g_local^T
-  const CDM_F64 pc = -(CDM_F64)b * CDM_NODDI_DPAR;
-  const CDM_F64 dx = (CDM_F64)gx, dy = (CDM_F64)gy, dn = (CDM_F64)gn;
-  CDM_F64 e[3];
-  bingham_eig3(dx * dx * pc, dx * dy * pc, dx * dn * pc, (CDM_F64)beta + dy * dy * pc, dy * dn * pc,
-               (CDM_F64)kappa + dn * dn * pc, e);
-  intra = (T)(bingham_hyp(e[0], e[1], e[2]) / f.nc);
+  const CDM_F64 r = (CDM_F64)b * CDM_NODDI_DPAR;
+  const CDM_F64 hyp = r > 0.0 ? bingham_hyp_rank1((CDM_F64)beta, (CDM_F64)kappa, r, (CDM_F64)gx, (CDM_F64)gy,
+                                                  (CDM_F64)gn)
+                              : bingham_hyp(0.0, (CDM_F64)beta, (CDM_F64)kappa);
+  intra = (T)(hyp / f.nc);
 }
 
 MACRO T bingham_fan_signal(T fintra, T kappa, T beta, T th, T ph, T psi, const bingham_fan& f, T* CFP) {
