@@ -441,8 +441,11 @@ struct cdm_jacobian_setup {
   CDM_F64 vcp[cdm_np][cdm_vc], vcm[cdm_np][cdm_vc];
   TJ h[cdm_np];
 #endif
-  __device__ __forceinline__ void init(TJ* p, TJ* fx) {
-    cdm_prepare(p, fx, vc0);
+  // Voxel caches at p and at every dependent probe.  When the group has at
+  // least as many lanes as caches, lane j computes cache j and the group
+  // exchanges them, instead of every lane computing all of them.
+  template <int L>
+  __device__ __forceinline__ void init(TJ* p, TJ* fx, int lane) {
 #ifdef CDM_NUMERICAL_MASK
 #pragma unroll
     for (int i = 0; i < cdm_np; ++i) {
@@ -450,15 +453,54 @@ struct cdm_jacobian_setup {
       TJ hi = fabs_gpu((TJ)::cdm_fd_step<TJ>::rel * p[i]);
       hi = min_gpu(max_gpu(hi, (TJ)::cdm_fd_step<TJ>::lo), (TJ)::cdm_fd_step<TJ>::hi);
       h[i] = hi;
-      if ((cdm_vc_dep >> i) & 1u) {
-        const TJ keep = p[i];
-        p[i] += hi * (TJ)0.5;
-        cdm_prepare(p, fx, vcp[i]);
-        p[i] -= (TJ)2 * (hi * (TJ)0.5);
-        cdm_prepare(p, fx, vcm[i]);
-        p[i] = keep;
-      }
     }
+    constexpr int njobs = 1 + 2 * __builtin_popcount(CDM_NUMERICAL_MASK & cdm_vc_dep);
+    if constexpr (cdm_vc > 0 && njobs > 1 && L >= njobs) {
+      // job 0: p; job 2k+1 / 2k+2: the k-th dependent probe p_i + h/2, p_i + h/2 - 2 (h/2)
+      const int job = lane < njobs ? lane : 0;
+      TJ q[cdm_np];
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i) q[i] = p[i];
+      int k = 0;
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i) {
+        if (!((CDM_NUMERICAL_MASK >> i) & 1u) || !((cdm_vc_dep >> i) & 1u)) continue;
+        const TJ up = p[i] + h[i] * (TJ)0.5;
+        if (job == 2 * k + 1) q[i] = up;
+        if (job == 2 * k + 2) q[i] = up - (TJ)2 * (h[i] * (TJ)0.5);
+        ++k;
+      }
+      CDM_F64 mine[cdm_vc];
+      cdm_prepare(q, fx, mine);
+#pragma unroll
+      for (int c = 0; c < cdm_vc; ++c) vc0[c] = cdm::Group<L>::bcast(mine[c], 0);
+      k = 0;
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i) {
+        if (!((CDM_NUMERICAL_MASK >> i) & 1u) || !((cdm_vc_dep >> i) & 1u)) continue;
+#pragma unroll
+        for (int c = 0; c < cdm_vc; ++c) {
+          vcp[i][c] = cdm::Group<L>::bcast(mine[c], 2 * k + 1);
+          vcm[i][c] = cdm::Group<L>::bcast(mine[c], 2 * k + 2);
+        }
+        ++k;
+      }
+      return;
+    }
+    cdm_prepare(p, fx, vc0);
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) {
+      if (!((CDM_NUMERICAL_MASK >> i) & 1u) || !((cdm_vc_dep >> i) & 1u)) continue;
+      const TJ keep = p[i];
+      p[i] += h[i] * (TJ)0.5;
+      cdm_prepare(p, fx, vcp[i]);
+      p[i] -= (TJ)2 * (h[i] * (TJ)0.5);
+      cdm_prepare(p, fx, vcm[i]);
+      p[i] = keep;
+    }
+#else
+    (void)lane;
+    cdm_prepare(p, fx, vc0);
 #endif
   }
   __device__ __forceinline__ TJ eval(TJ* p, TJ* cm, TJ* fx, const CDM_F64* mc, TJ* d) {
@@ -540,7 +582,7 @@ __device__ __forceinline__ void cdm_levmar_body(const cdm_lm_args& a, int marqua
       for (int k = 0; k < cdm_ntri; ++k) hs[k] = 0;
       cdm_cfp_row<cdm_jt> rowj;
       cdm_jacobian_setup<cdm_jt> jac;
-      jac.init(pj, fxj);
+      jac.template init<L>(pj, fxj, lane);
       for (int m = lane; m < a.g.nmeas; m += L) {
         cdm_jt* cm = rowj.at(s_cfp, m, a.g.cfp_ts);
         cdm_jt d[cdm_np];

@@ -75,6 +75,20 @@ struct Group {
   }
   // Logical OR over the group (flags kept as 0/1 ints).
   __device__ __forceinline__ static int any(int f) { return sum(f) != 0; }
+  // The value held by group lane `src` (0 <= src < L), in every lane of the
+  // group (ds_bpermute from lane (first lane of the group) + src).
+  __device__ __forceinline__ static int bcast(int v, int src) {
+    if constexpr (L == 1) return v;
+    const int from = (int)(__lane_id() & ~(unsigned)(L - 1)) + src;
+    return __builtin_amdgcn_ds_bpermute(from << 2, v);
+  }
+  __device__ __forceinline__ static float bcast(float v, int src) { return __int_as_float(bcast(__float_as_int(v), src)); }
+  __device__ __forceinline__ static double bcast(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const int lo = bcast((int)(b & 0xffffffffll), src);
+    const int hi = bcast((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
 };
 
 }  // namespace cdm
