@@ -91,7 +91,8 @@ def _compile(src, obj, deps, extra=()):
 DEFAULT_BATCHES = {"tensor_bingham2_sticks3": ["DiffusionTensor", "NODDI_Bingham_2Fans", "Ball_3_Sticks"]}
 
 PLUGIN_MACROS = ["MODEL_H_INCLUDED", "NPARAMS", "NCFP", "NFIXP", "CDM_LM_JACOBIAN_DOUBLE", "CDM_VOXEL_CACHE",
-                 "CDM_VOXEL_CACHE_DEPENDS", "CDM_MEAS_CACHE", "CDM_NUMERICAL_MASK"]
+                 "CDM_VOXEL_CACHE_DEPENDS", "CDM_MEAS_CACHE", "CDM_NUMERICAL_MASK", "CDM_MEAS_VOXEL_CACHE",
+                 "CDM_MEAS_VOXEL_CACHE_DEPENDS"]
 
 
 def batch_unit_source(name, model_dirs):

@@ -54,6 +54,7 @@ static cdm_mcmc_args op_mcmc_args(const cdm::Geom& g, const cdm::McmcLaunch& m, 
   cdm_mcmc_args a;
   a.g = op_geom(g);
   a.params = m.params;
+  a.mvc = (cdm_tf*)m.mvc;
   a.state = (cdm_chain*)m.state;
   a.samples = m.samples;
   a.tau_samples = m.tau_samples;

@@ -42,6 +42,13 @@ constexpr int cdm_ntri = NPARAMS * (NPARAMS + 1) / 2;
 //   CDM_MEAS_CACHE k           k values per measurement from Meas_Cache(CFP,
 //                              mc), computed once per block into LDS
 //   Predicted_Signal_Cached(npar, P, CFP, FixP, vc, mc)
+//   CDM_MEAS_VOXEL_CACHE k     k values per (voxel, measurement) that depend
+//   CDM_MEAS_VOXEL_CACHE_DEPENDS b  only on the parameters in bit mask b,
+//                              from Meas_Voxel_Cache(npar, P, CFP, FixP, vc,
+//                              mc, mvc); Predicted_Signal_MV(npar, P, CFP,
+//                              FixP, vc, mc, mvc) uses them.  MCMC keeps them
+//                              per voxel so that a step in another parameter
+//                              reuses them
 //   CDM_NUMERICAL_MASK b       derivatives the engine forms by central
 //                              differences (the reference's NUMERICAL step
 //                              rule) with per-perturbation voxel caches;
@@ -58,6 +65,13 @@ constexpr unsigned cdm_vc_dep = 0;
 constexpr int cdm_mc = CDM_MEAS_CACHE;
 #else
 constexpr int cdm_mc = 0;
+#endif
+#ifdef CDM_MEAS_VOXEL_CACHE
+constexpr int cdm_mvc = CDM_MEAS_VOXEL_CACHE;
+constexpr unsigned cdm_mvc_dep = CDM_MEAS_VOXEL_CACHE_DEPENDS;
+#else
+constexpr int cdm_mvc = 0;
+constexpr unsigned cdm_mvc_dep = 0;
 #endif
 
 template <class TF>
@@ -726,6 +740,7 @@ struct cdm_chain {  // FP64 chain state per voxel
 struct cdm_mcmc_args {
   cdm_geom g;
   float* params;         // [nvox][NP] in/out
+  cdm_tf* mvc;           // [nvox][2][nmeas][cdm_mvc] per-measurement caches (cdm_mvc > 0)
   cdm_chain* state;      // [nvox]
   float* samples;        // [nvox][NP][nsamples] (recording)
   float* tau_samples;    // [nvox][nsamples] (Rician, recording)
@@ -780,6 +795,48 @@ __device__ __forceinline__ double cdm_neg_loglik(const cdm_geom& g, const float*
   acc = cdm::Group<L>::sum(acc);
   if constexpr (RICIAN) return -g.nmeas * log(tau) - acc;
   else return (g.nmeas / 2.0) * log(acc / 2.0);
+}
+
+// The same with the per-measurement caches: MODE 1 reads them from `row`
+// ([nmeas][cdm_mvc] of this voxel), MODE 2 computes them, stores them in
+// `row` and uses them.  Each lane reads only what it wrote itself.
+template <int L, bool RICIAN, int MODE>
+__device__ __forceinline__ double cdm_neg_loglik_mv(const cdm_geom& g, const float* s_cfp, const CDM_F64* s_mc,
+                                                    const float* mv, const double* p, double tau, cdm_tf* fx,
+                                                    const CDM_F64* vc, cdm_tf* row, int lane) {
+#ifdef CDM_MEAS_VOXEL_CACHE
+  cdm_tf pf[cdm_np];
+#pragma unroll
+  for (int i = 0; i < cdm_np; ++i) pf[i] = (cdm_tf)p[i];
+  cdm_cfp_row<cdm_tf> crow;
+  double acc = 0.0;
+  for (int m = lane; m < g.nmeas; m += L) {
+    cdm_tf* cfpm = crow.at(s_cfp, m, g.cfp_ts);
+    const CDM_F64* mcm = cdm_mc_row(s_mc, m);
+    cdm_tf mvc[cdm_mvc];
+    if constexpr (MODE == 2) {
+      Meas_Voxel_Cache(cdm_np, pf, cfpm, fx, vc, mcm, mvc);
+#pragma unroll
+      for (int k = 0; k < cdm_mvc; ++k) row[m * cdm_mvc + k] = mvc[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < cdm_mvc; ++k) mvc[k] = row[m * cdm_mvc + k];
+    }
+    const cdm_tf s = Predicted_Signal_MV(cdm_np, pf, cfpm, fx, vc, mcm, mvc);
+    if constexpr (RICIAN) {
+      const double y = mv[m], sd = (double)s;
+      acc += log(y) + (-0.5 * tau * (y * y + sd * sd) + cdm_logI0(tau * sd * y));
+    } else {
+      const cdm_tf r = s - (cdm_tf)mv[m];
+      acc += (double)(r * r);
+    }
+  }
+  acc = cdm::Group<L>::sum(acc);
+  if constexpr (RICIAN) return -g.nmeas * log(tau) - acc;
+  else return (g.nmeas / 2.0) * log(acc / 2.0);
+#else
+  return cdm_neg_loglik<L, RICIAN>(g, s_cfp, s_mc, mv, p, tau, fx, vc, lane);
+#endif
 }
 
 // Voxel cache of a double parameter vector, computed in the plug-in's precision.
@@ -903,6 +960,11 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
   // voxel cache of the current state (a deterministic function of p)
   CDM_F64 vc_cur[cdm_vc];
   cdm_prepare_d(p, fx, vc_cur);
+  // per-measurement caches: two rows per voxel, `cur` holds the current state's
+  cdm_tf* mvc_rows = cdm_mvc ? a.mvc + (size_t)v * 2 * a.g.nmeas * cdm_mvc : nullptr;
+  int cur = 0;
+  if constexpr (cdm_mvc > 0)
+    (void)cdm_neg_loglik_mv<L, RICIAN, 2>(a.g, s_cfp, s_mc, mv, p, tau, fx, vc_cur, mvc_rows, lane);
   const int end = a.begin + a.count;
   for (int it = a.begin; it < end; ++it) {
     const uint32_t jump = a.jump_offset + (uint32_t)it;
@@ -914,7 +976,10 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
         double tp = 0.0;
 #pragma unroll
         for (int i = 0; i < cdm_np; ++i) tp += prior[i];
-        const double e_new = tp + cdm_neg_loglik<L, true>(a.g, s_cfp, s_mc, mv, p, tau, fx, vc_cur, lane);
+        const double e_new =
+            tp + (cdm_mvc > 0 ? cdm_neg_loglik_mv<L, true, 1>(a.g, s_cfp, s_mc, mv, p, tau, fx, vc_cur,
+                                                              mvc_rows + cur * a.g.nmeas * cdm_mvc, lane)
+                              : cdm_neg_loglik<L, true>(a.g, s_cfp, s_mc, mv, p, tau, fx, vc_cur, lane));
         if (exp(energy - e_new) > d.uniform) {
           ++tau_acc;
           energy = e_new;
@@ -939,9 +1004,20 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
       if (ok) {
         // the voxel cache changes only with the parameters it depends on
         const bool dep = (cdm_vc_dep >> i) & 1u;
+        const bool mdep = (cdm_mvc_dep >> i) & 1u;
         CDM_F64 vc_new[cdm_vc];
         if (dep) cdm_prepare_d(p, fx, vc_new);
-        const double like = cdm_neg_loglik<L, RICIAN>(a.g, s_cfp, s_mc, mv, p, tau, fx, dep ? vc_new : vc_cur, lane);
+        const CDM_F64* vcu = dep ? vc_new : vc_cur;
+        double like;
+        if constexpr (cdm_mvc > 0) {
+          const size_t rs = (size_t)a.g.nmeas * cdm_mvc;
+          like = mdep ? cdm_neg_loglik_mv<L, RICIAN, 2>(a.g, s_cfp, s_mc, mv, p, tau, fx, vcu,
+                                                        mvc_rows + (1 - cur) * rs, lane)
+                      : cdm_neg_loglik_mv<L, RICIAN, 1>(a.g, s_cfp, s_mc, mv, p, tau, fx, vcu, mvc_rows + cur * rs,
+                                                        lane);
+        } else {
+          like = cdm_neg_loglik<L, RICIAN>(a.g, s_cfp, s_mc, mv, p, tau, fx, vcu, lane);
+        }
         const double old_prior = prior[i];
         prior[i] = cdm_prior(a.c, i, p, a.g.nmeas, s_cfp, fx);
         double tp = 0.0;
@@ -955,6 +1031,7 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
 #pragma unroll
             for (int k = 0; k < cdm_vc; ++k) vc_cur[k] = vc_new[k];
           }
+          if (mdep) cur = 1 - cur;
         } else {
           ++nrej[i];
           p[i] = old;

@@ -244,7 +244,14 @@ int upload_grid(cdm_model* m) {
   return 0;
 }
 
-int ensure_state(cdm_model* m, size_t bytes) {
+// MCMC chain state of nvox voxels, followed (256-byte aligned) by the two
+// per-measurement cache rows of each voxel when the plug-in declares them.
+size_t mvc_offset(const cdm_model* m, int nvox) {
+  return (m->ops->chain_bytes * (size_t)nvox + 255) / 256 * 256;
+}
+
+int ensure_state(cdm_model* m, int nvox, int nmeas) {
+  const size_t bytes = mvc_offset(m, nvox) + m->ops->mvc_bytes_per_meas * 2 * (size_t)nmeas * (size_t)nvox;
   if (m->state_bytes >= bytes) return 0;
   if (m->d_state) (void)hipFree(m->d_state);
   m->d_state = nullptr;
@@ -253,6 +260,10 @@ int ensure_state(cdm_model* m, size_t bytes) {
   if (e != hipSuccess) return hip_fail(e, "MCMC chain state");
   m->state_bytes = bytes;
   return 0;
+}
+
+void* mvc_rows(cdm_model* m, int nvox) {
+  return m->ops->mvc_bytes_per_meas ? (void*)((char*)m->d_state + mvc_offset(m, nvox)) : nullptr;
 }
 
 // Records the launches that `issue` makes on a private capture stream into a
@@ -464,7 +475,7 @@ int cdm_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params
     return fail("--nj must be a multiple of --se (the reference writes past the sample buffer otherwise)");
   const int nb = nburn < 1 ? 1 : nburn;  // MCMC.cu:702-703
   const int nsamples = njumps / sampleevery;
-  if ((r = ensure_state(m, m->ops->chain_bytes * (size_t)nvox))) return r;
+  if ((r = ensure_state(m, nvox, nmeas))) return r;
   const cdm::Geom g = make_geom(m, nvox, nmeas, meas, cfp, fixp);
   const cdm::Consts c = make_consts(m);
   const int L = group_for(m, cdm::kMcmc, nmeas, nvox);
@@ -473,6 +484,7 @@ int cdm_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params
   cdm::McmcLaunch ml;
   ml.params = params;
   ml.state = m->d_state;
+  ml.mvc = mvc_rows(m, nvox);
   ml.samples = samples;
   ml.tau_samples = tau_samples;
   ml.nsamples = nsamples;
@@ -705,7 +717,7 @@ int cdm_batch_mcmc(int nitems, const cdm_batch_item* items, int nburn, int njump
   const int nsamples = njumps / sampleevery;
   hipStream_t s = (hipStream_t)stream;
   for (int i = 0; i < nitems; ++i)
-    if ((r = ensure_state(items[i].model, items[i].model->ops->chain_bytes * (size_t)items[i].nvox))) return r;
+    if ((r = ensure_state(items[i].model, items[i].nvox, items[i].nmeas))) return r;
   const std::vector<int> order = batch_order(nitems, items, cdm::kMcmc);
   // one prepared item set per launch: phases x jump blocks
   struct Launch {
@@ -735,6 +747,7 @@ int cdm_batch_mcmc(int nitems, const cdm_batch_item* items, int nburn, int njump
       cdm::McmcLaunch& ml = b.mc;
       ml.params = it.params;
       ml.state = it.model->d_state;
+      ml.mvc = mvc_rows(it.model, it.nvox);
       ml.samples = it.samples;
       ml.tau_samples = it.tau_samples;
       ml.nsamples = nsamples;

@@ -29,6 +29,7 @@ struct Geom {
 struct McmcLaunch {
   float* params;
   void* state;
+  void* mvc;  // per-measurement cache rows (mvc_bytes_per_meas x 2 x nmeas per voxel)
   float* samples;
   float* tau_samples;
   int nsamples, sampleevery, upe;
@@ -50,6 +51,7 @@ struct ModelOps {
   const int* fixp_sizes;
   int forward_is_double;
   size_t chain_bytes;
+  size_t mvc_bytes_per_meas;  // per-measurement cache of one state (0: none)
   int cfp_max, fixp_max;
   hipError_t (*gridsearch)(int L, const Geom& g, float* params, int ngp, const int* gparams, int ncombs,
                            const float* grid_f, const double* grid_d, const int* comb_id, const Consts& c,

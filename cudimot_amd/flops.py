@@ -10,9 +10,11 @@ counted (not estimated) by running each plug-in on the CPU with an
 operation-counting scalar (lib/libcdm_opcount.so, csrc/tools/opcount.h):
 +, -, * = 1, / and sqrt = 1, each transcendental = 1, FMA = 2.  Method
 totals per voxel:
-  MCMC  = jumps * P * M * (F_fwd + 3 FP32 + 1 FP64)   (residual, square,
-          FP64 accumulation per measurement; proposals and priors are O(P)
-          per jump and not counted)
+  MCMC  = jumps * sum_params M * (F_meas + 3 FP32 + 1 FP64)  (residual,
+          square, FP64 accumulation per measurement; F_meas is the cached
+          per-measurement signal, plus the per-measurement cache when the
+          proposed parameter is one it depends on; proposals and priors are
+          O(P) per jump and not counted) + voxel caches of dependent proposals
   LM    = sum over iterations of M * (F_fwd + F_jac + P(P+1)/2 * 2 + 2P)
           for accepted steps plus M * F_fwd per cost evaluation
   GS    = combinations * M * (F_fwd + 3)
@@ -42,24 +44,35 @@ def per_eval(model, params, cfp, fixp=None):
     cfp = np.ascontiguousarray(cfp, dtype=np.float64)
     n = params.shape[0]
     fixp = np.ascontiguousarray(fixp if fixp is not None else np.zeros((n, 1)), dtype=np.float64)
-    out = np.zeros(14)
+    out = np.zeros(20)
     D = ctypes.POINTER(ctypes.c_double)
     getattr(_load(), "cdm_opcount_" + model)(n, cfp.shape[0], params.ctypes.data_as(D), cfp.ctypes.data_as(D),
                                              fixp.ctypes.data_as(D), out.ctypes.data_as(D))
     return {"fwd32": out[0], "fwd64": out[1], "fwd_t32": out[2], "fwd_t64": out[3],
             "jac32": out[4], "jac64": out[5], "jac_t32": out[6], "jac_t64": out[7],
             "vc32": out[8], "vc64": out[9], "vc_dep": int(out[10]), "cached": bool(out[11]),
-            "meas32": out[12], "meas64": out[13]}
+            "meas32": out[12], "meas64": out[13],
+            "mv32": out[14], "mv64": out[15], "pmv32": out[16], "pmv64": out[17], "mvc_dep": int(out[18]),
+            "mvc": bool(out[19])}
 
 
 def mcmc_flops(ev, nparams, nmeas, jumps):
     """(FP32, FP64) flops of one voxel's MCMC over `jumps` jumps: every
-    proposal evaluates the signal at all measurements, and recomputes the
-    voxel cache when the proposed parameter is one it depends on."""
-    n = jumps * nparams * nmeas
+    proposal evaluates the signal at all measurements, recomputes the voxel
+    cache when the proposed parameter is one it depends on, and (with
+    per-measurement caches) recomputes those only when it depends on them."""
     ndep = bin(ev["vc_dep"]).count("1") if ev["cached"] else 0
-    f32 = n * (ev["meas32"] + 3) + jumps * ndep * ev["vc32"]
-    f64 = n * (ev["meas64"] + 1) + jumps * ndep * ev["vc64"]
+    f32 = jumps * ndep * ev["vc32"]
+    f64 = jumps * ndep * ev["vc64"]
+    for i in range(nparams):
+        if ev.get("mvc"):
+            dep = (ev["mvc_dep"] >> i) & 1
+            m32 = ev["pmv32"] + (ev["mv32"] if dep else 0.0)
+            m64 = ev["pmv64"] + (ev["mv64"] if dep else 0.0)
+        else:
+            m32, m64 = ev["meas32"], ev["meas64"]
+        f32 += jumps * nmeas * (m32 + 3)
+        f64 += jumps * nmeas * (m64 + 1)
     return f32, f64
 
 

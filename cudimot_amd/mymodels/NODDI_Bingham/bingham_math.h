@@ -129,11 +129,10 @@ __device__ inline void bingham_fan_setup(CDM_F64 fintra, CDM_F64 kappa, CDM_F64 
   f.dperp2 = CDM_NODDI_DPAR + 2 * dperp - f.dpar - f.dperp1;
 }
 
-// Hindered (e) and intra-axonal (i) signals of one fan for gradient g and
-// b-value b.  Frame: fibre n(th, ph), fanning direction from psi, and their
-// cross product (NODDI_Bingham/modelfunctions.h:116-138).
-MACRO void bingham_fan_signal_parts(T kappa, T beta, T th, T ph, T psi, const bingham_fan& f, T* CFP, T& extra,
-                                    T& intra) {
+// Gradient in the fan's local frame for measurement CFP (g, b): components
+// along the fibre n(th, ph), the fanning direction (from psi) and their cross
+// product (NODDI_Bingham/modelfunctions.h:116-138).
+MACRO void bingham_fan_local(T th, T ph, T psi, T* CFP, T& gx, T& gy, T& gn) {
   T st, ct, sp, cp, ss, cs;
   tsincos(th, &st, &ct);
   tsincos(ph, &sp, &cp);
@@ -141,22 +140,49 @@ MACRO void bingham_fan_signal_parts(T kappa, T beta, T th, T ph, T psi, const bi
   const T n0 = st * cp, n1 = st * sp, n2 = ct;
   const T f0 = -cs * sp - ct * cp * ss, f1 = cp * cs - ct * sp * ss, f2 = st * ss;
   const T gx_local0 = f1 * n2 - f2 * n1, gx_local1 = f2 * n0 - f0 * n2, gx_local2 = f0 * n1 - f1 * n0;
-  const T gn = CFP[0] * n0 + CFP[1] * n1 + CFP[2] * n2;       // along the fibre
-  const T gy = CFP[0] * f0 + CFP[1] * f1 + CFP[2] * f2;       // along the fanning direction
-  const T gx = CFP[0] * gx_local0 + CFP[1] * gx_local1 + CFP[2] * gx_local2;
-  const T b = CFP[3];
-  // hindered compartment
+  gn = CFP[0] * n0 + CFP[1] * n1 + CFP[2] * n2;       // along the fibre
+  gy = CFP[0] * f0 + CFP[1] * f1 + CFP[2] * f2;       // along the fanning direction
+  gx = CFP[0] * gx_local0 + CFP[1] * gx_local1 + CFP[2] * gx_local2;
+}
+
+// Hindered compartment of one fan from the local gradient.
+MACRO T bingham_fan_extra_local(T gx, T gy, T gn, T b, const bingham_fan& f) {
   const T c2 = gn * gn, s2 = (T)1 - c2;
   // cos^2 of the azimuth atan2(gy, gx) in the plane normal to the fibre
   const T gxy = gx * gx + gy * gy;
   const T caz2 = gxy > (T)0 ? gx * gx / gxy : (T)1, saz2 = (T)1 - caz2;
-  extra = exp_gpu(-b * ((T)f.dpar * c2 + s2 * ((T)f.dperp1 * saz2 + (T)f.dperp2 * caz2)));
-  // intra-axonal compartment: 1F1 of diag(0, beta, kappa) - b Dpar g_local g_local^T
+  return exp_gpu(-b * ((T)f.dpar * c2 + s2 * ((T)f.dperp1 * saz2 + (T)f.dperp2 * caz2)));
+}
+
+// Intra-axonal compartment of one fan from the local gradient: 1F1 of
+// diag(0, beta, kappa) - b Dpar g_local g_local^T over 1F1(kappa, beta, 0).
+MACRO T bingham_fan_intra_local(T kappa, T beta, T gx, T gy, T gn, T b, const bingham_fan& f) {
   const CDM_F64 r = (CDM_F64)b * CDM_NODDI_DPAR;
   const CDM_F64 hyp = r > 0.0 ? bingham_hyp_rank1((CDM_F64)beta, (CDM_F64)kappa, r, (CDM_F64)gx, (CDM_F64)gy,
                                                   (CDM_F64)gn)
                               : bingham_hyp(0.0, (CDM_F64)beta, (CDM_F64)kappa);
-  intra = (T)(hyp / f.nc);
+  return (T)(hyp / f.nc);
+}
+
+MACRO T bingham_fan_extra(T th, T ph, T psi, const bingham_fan& f, T* CFP) {
+  T gx, gy, gn;
+  bingham_fan_local(th, ph, psi, CFP, gx, gy, gn);
+  return bingham_fan_extra_local(gx, gy, gn, CFP[3], f);
+}
+
+MACRO T bingham_fan_intra(T kappa, T beta, T th, T ph, T psi, const bingham_fan& f, T* CFP) {
+  T gx, gy, gn;
+  bingham_fan_local(th, ph, psi, CFP, gx, gy, gn);
+  return bingham_fan_intra_local(kappa, beta, gx, gy, gn, CFP[3], f);
+}
+
+// Hindered (e) and intra-axonal (i) signals of one fan.
+MACRO void bingham_fan_signal_parts(T kappa, T beta, T th, T ph, T psi, const bingham_fan& f, T* CFP, T& extra,
+                                    T& intra) {
+  T gx, gy, gn;
+  bingham_fan_local(th, ph, psi, CFP, gx, gy, gn);
+  extra = bingham_fan_extra_local(gx, gy, gn, CFP[3], f);
+  intra = bingham_fan_intra_local(kappa, beta, gx, gy, gn, CFP[3], f);
 }
 
 MACRO T bingham_fan_signal(T fintra, T kappa, T beta, T th, T ph, T psi, const bingham_fan& f, T* CFP) {

@@ -9,12 +9,16 @@
 // Engine cache hooks: the normaliser 1F1(kappa, beta, 0) and the hindered
 // diffusivities (five 1F1 evaluations) depend on fintra, kappa and beta
 // only, so they are computed once per parameter vector (Voxel_Cache) instead
-// of once per measurement.
+// of once per measurement; the intra-axonal signal of each measurement does
+// not depend on fiso or fintra, so an MCMC step in those two reuses it
+// (Meas_Voxel_Cache, Predicted_Signal_MV).
 #include "bingham_math.h"
 
 #define CDM_VOXEL_CACHE 4
-#define CDM_VOXEL_CACHE_DEPENDS 0x0Eu  // fintra, kappa, beta
-#define CDM_NUMERICAL_MASK 0x7Cu       // kappa, beta, th, ph, psi
+#define CDM_VOXEL_CACHE_DEPENDS 0x0Eu       // fintra, kappa, beta
+#define CDM_NUMERICAL_MASK 0x7Cu            // kappa, beta, th, ph, psi
+#define CDM_MEAS_VOXEL_CACHE 1              // intra-axonal signal per measurement
+#define CDM_MEAS_VOXEL_CACHE_DEPENDS 0x7Cu  // kappa, beta, th, ph, psi
 
 MACRO void Voxel_Cache(int npar, T* P, T* FixP, CDM_F64* vc) {
   bingham_fan f;
@@ -38,6 +42,27 @@ MACRO void bingham1_parts(T* P, T* CFP, const CDM_F64* vc, T& fan, T& e, T& i) {
 MACRO T Predicted_Signal_Cached(int npar, T* P, T* CFP, T* FixP, const CDM_F64* vc, const CDM_F64* mc) {
   T fan, e, i;
   bingham1_parts(P, CFP, vc, fan, e, i);
+  const T iso = exp_gpu(-CFP[3] * (T)CDM_NODDI_DISO);
+  return FixP[0] * (((T)1 - P[0]) * fan + P[0] * iso);
+}
+
+MACRO void Meas_Voxel_Cache(int npar, T* P, T* CFP, T* FixP, const CDM_F64* vc, const CDM_F64* mc, T* mvc) {
+  bingham_fan f;
+  f.nc = vc[0];
+  f.dpar = vc[1];
+  f.dperp1 = vc[2];
+  f.dperp2 = vc[3];
+  mvc[0] = bingham_fan_intra(P[2], P[3], P[4], P[5], P[6], f, CFP);
+}
+
+MACRO T Predicted_Signal_MV(int npar, T* P, T* CFP, T* FixP, const CDM_F64* vc, const CDM_F64* mc, const T* mvc) {
+  bingham_fan f;
+  f.nc = vc[0];
+  f.dpar = vc[1];
+  f.dperp1 = vc[2];
+  f.dperp2 = vc[3];
+  const T e = bingham_fan_extra(P[4], P[5], P[6], f, CFP);
+  const T fan = ((T)1 - P[1]) * e + P[1] * mvc[0];
   const T iso = exp_gpu(-CFP[3] * (T)CDM_NODDI_DISO);
   return FixP[0] * (((T)1 - P[0]) * fan + P[0] * iso);
 }
