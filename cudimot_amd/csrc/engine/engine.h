@@ -81,6 +81,39 @@ __device__ __forceinline__ void cdm_prepare(TF* p, TF* fx, CDM_F64* vc) {
 #endif
 }
 
+#ifdef CDM_VOXEL_CACHE_JOBS
+constexpr int cdm_vc_jobs = CDM_VOXEL_CACHE_JOBS;
+#else
+constexpr int cdm_vc_jobs = 0;
+#endif
+
+// The voxel cache computed by a group of L lanes.  A plug-in that declares
+// CDM_VOXEL_CACHE_JOBS n splits Voxel_Cache into n independent scalar jobs
+// (Voxel_Cache_Job) and a combination (Voxel_Cache_Combine); lane l computes
+// jobs l, l+L, ..., the group exchanges the results and every lane combines
+// them, with the arithmetic of Voxel_Cache.
+template <int L, class TF>
+__device__ __forceinline__ void cdm_prepare_g(TF* p, TF* fx, CDM_F64* vc, int lane) {
+#if defined(CDM_VOXEL_CACHE) && defined(CDM_VOXEL_CACHE_JOBS)
+  if constexpr (L > 1) {
+    constexpr int rounds = (cdm_vc_jobs + L - 1) / L;
+    CDM_F64 mine[rounds];
+#pragma unroll
+    for (int r = 0; r < rounds; ++r) {
+      const int j = lane + r * L;
+      mine[r] = Voxel_Cache_Job(j < cdm_vc_jobs ? j : 0, cdm_np, p, fx);
+    }
+    CDM_F64 jr[cdm_vc_jobs];
+#pragma unroll
+    for (int j = 0; j < cdm_vc_jobs; ++j) jr[j] = cdm::Group<L>::bcast(mine[j / L], j % L);
+    Voxel_Cache_Combine(cdm_np, p, fx, jr, vc);
+    return;
+  }
+#endif
+  (void)lane;
+  cdm_prepare(p, fx, vc);
+}
+
 template <class TF>
 __device__ __forceinline__ TF cdm_signal(TF* p, TF* cfp, TF* fx, const CDM_F64* vc, const CDM_F64* mc) {
 #ifdef CDM_VOXEL_CACHE
@@ -168,7 +201,7 @@ __device__ __forceinline__ double cdm_sse(const cdm_geom& g, const float* s_cfp,
   double acc = 0.0;
   cdm_cfp_row<TF> row;
   CDM_F64 vc[cdm_vc];
-  cdm_prepare(p, fx, vc);
+  cdm_prepare_g<L>(p, fx, vc, lane);
   for (int m = lane; m < g.nmeas; m += L) {
     const TF s = cdm_signal(p, row.at(s_cfp, m, g.cfp_ts), fx, vc, cdm_mc_row(s_mc, m));
     const TF r = s - (TF)mv[m];
@@ -455,6 +488,36 @@ struct cdm_jacobian_setup {
   CDM_F64 vcp[cdm_np][cdm_vc], vcm[cdm_np][cdm_vc];
   TJ h[cdm_np];
 #endif
+#ifdef CDM_NUMERICAL_MASK
+  // Parameters of cache c: 0 = p, 2k+1 / 2k+2 = the k-th dependent probe
+  // p_i + h/2, p_i + h/2 - 2 (h/2) (the probes of eval()).
+  __device__ __forceinline__ void probe_params(const TJ* p, int c, TJ* q) const {
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) q[i] = p[i];
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) {
+      if (!((CDM_NUMERICAL_MASK >> i) & 1u) || !((cdm_vc_dep >> i) & 1u)) continue;
+      const TJ up = p[i] + h[i] * (TJ)0.5;
+      if (c == 2 * k + 1) q[i] = up;
+      if (c == 2 * k + 2) q[i] = up - (TJ)2 * (h[i] * (TJ)0.5);
+      ++k;
+    }
+  }
+  // Storage of cache c (compile-time c after unrolling).
+  __device__ __forceinline__ CDM_F64* cache_of(int c) {
+    if (c == 0) return vc0;
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) {
+      if (!((CDM_NUMERICAL_MASK >> i) & 1u) || !((cdm_vc_dep >> i) & 1u)) continue;
+      if (c == 2 * k + 1) return vcp[i];
+      if (c == 2 * k + 2) return vcm[i];
+      ++k;
+    }
+    return vc0;
+  }
+#endif
   // Voxel caches at p and at every dependent probe.  When the group has at
   // least as many lanes as caches, lane j computes cache j and the group
   // exchanges them, instead of every lane computing all of them.
@@ -469,6 +532,33 @@ struct cdm_jacobian_setup {
       h[i] = hi;
     }
     constexpr int njobs = 1 + 2 * __builtin_popcount(CDM_NUMERICAL_MASK & cdm_vc_dep);
+#ifdef CDM_VOXEL_CACHE_JOBS
+    if constexpr (cdm_vc > 0 && njobs > 1 && L >= 2) {
+      // every (cache, job) pair on its own lane: pair t = c * jobs + j
+      constexpr int T = njobs * cdm_vc_jobs, rounds = (T + L - 1) / L;
+      CDM_F64 mine[rounds];
+#pragma unroll
+      for (int r = 0; r < rounds; ++r) {
+        int t = lane + r * L;
+        if (t >= T) t = 0;
+        TJ q[cdm_np];
+        probe_params(p, t / cdm_vc_jobs, q);
+        mine[r] = Voxel_Cache_Job(t % cdm_vc_jobs, cdm_np, q, fx);
+      }
+#pragma unroll
+      for (int c = 0; c < njobs; ++c) {
+        CDM_F64 jr[cdm_vc_jobs];
+#pragma unroll
+        for (int j = 0; j < cdm_vc_jobs; ++j)
+          jr[j] = cdm::Group<L>::bcast(mine[(c * cdm_vc_jobs + j) / L], (c * cdm_vc_jobs + j) % L);
+        TJ q[cdm_np];
+        probe_params(p, c, q);
+        CDM_F64* dst = cache_of(c);
+        Voxel_Cache_Combine(cdm_np, q, fx, jr, dst);
+      }
+      return;
+    }
+#endif
     if constexpr (cdm_vc > 0 && njobs > 1 && L >= njobs) {
       // job 0: p; job 2k+1 / 2k+2: the k-th dependent probe p_i + h/2, p_i + h/2 - 2 (h/2)
       const int job = lane < njobs ? lane : 0;
@@ -846,6 +936,13 @@ __device__ __forceinline__ void cdm_prepare_d(const double* p, cdm_tf* fx, CDM_F
   for (int i = 0; i < cdm_np; ++i) pf[i] = (cdm_tf)p[i];
   cdm_prepare(pf, fx, vc);
 }
+template <int L>
+__device__ __forceinline__ void cdm_prepare_dg(const double* p, cdm_tf* fx, CDM_F64* vc, int lane) {
+  cdm_tf pf[cdm_np];
+#pragma unroll
+  for (int i = 0; i < cdm_np; ++i) pf[i] = (cdm_tf)p[i];
+  cdm_prepare_g<L>(pf, fx, vc, lane);
+}
 
 template <class TP>
 __device__ __forceinline__ double cdm_prior(const cdm_consts& c, int i, TP* p, int nmeas, float* cfp, cdm_tf* fx) {
@@ -959,7 +1056,7 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
 
   // voxel cache of the current state (a deterministic function of p)
   CDM_F64 vc_cur[cdm_vc];
-  cdm_prepare_d(p, fx, vc_cur);
+  cdm_prepare_dg<L>(p, fx, vc_cur, lane);
   // per-measurement caches: two rows per voxel, `cur` holds the current state's
   cdm_tf* mvc_rows = cdm_mvc ? a.mvc + (size_t)v * 2 * a.g.nmeas * cdm_mvc : nullptr;
   int cur = 0;
@@ -1006,7 +1103,7 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
         const bool dep = (cdm_vc_dep >> i) & 1u;
         const bool mdep = (cdm_mvc_dep >> i) & 1u;
         CDM_F64 vc_new[cdm_vc];
-        if (dep) cdm_prepare_d(p, fx, vc_new);
+        if (dep) cdm_prepare_dg<L>(p, fx, vc_new, lane);
         const CDM_F64* vcu = dep ? vc_new : vc_cur;
         double like;
         if constexpr (cdm_mvc > 0) {

@@ -117,16 +117,31 @@ struct bingham_fan {
   CDM_F64 dpar, dperp1, dperp2;    // hindered diffusivities
 };
 
-__device__ inline void bingham_fan_setup(CDM_F64 fintra, CDM_F64 kappa, CDM_F64 beta, bingham_fan& f) {
+// The five normaliser evaluations of a fan, one per job: 1F1(kappa, beta, 0)
+// and its central-difference probes in kappa and in beta (delta 5e-5).
+constexpr double bingham_fd_delta = 0.00005;
+__device__ inline CDM_F64 bingham_fan_job(int job, CDM_F64 kappa, CDM_F64 beta) {
+  const CDM_F64 delta = bingham_fd_delta;
+  const CDM_F64 k = job == 1 ? kappa + delta : job == 2 ? kappa - delta : kappa;
+  const CDM_F64 b = job == 3 ? beta + delta : job == 4 ? beta - delta : beta;
+  return bingham_hyp(k, b, 0.0);
+}
+
+// Fan quantities from the five job values h[0..4].
+__device__ inline void bingham_fan_combine(CDM_F64 fintra, const CDM_F64* h, bingham_fan& f) {
   const CDM_F64 dperp = CDM_NODDI_DPAR * (1.0 - fintra);
   const CDM_F64 dd = CDM_NODDI_DPAR - dperp;
-  const CDM_F64 delta = 0.00005;
-  f.nc = bingham_hyp(kappa, beta, 0.0);
-  CDM_F64 diff = (bingham_hyp(kappa + delta, beta, 0.0) - bingham_hyp(kappa - delta, beta, 0.0)) / (2.0 * delta);
-  f.dpar = dperp + dd * diff / f.nc;
-  diff = (bingham_hyp(kappa, beta + delta, 0.0) - bingham_hyp(kappa, beta - delta, 0.0)) / (2.0 * delta);
-  f.dperp1 = dperp + dd * diff / f.nc;
+  const CDM_F64 delta = bingham_fd_delta;
+  f.nc = h[0];
+  f.dpar = dperp + dd * ((h[1] - h[2]) / (2.0 * delta)) / f.nc;
+  f.dperp1 = dperp + dd * ((h[3] - h[4]) / (2.0 * delta)) / f.nc;
   f.dperp2 = CDM_NODDI_DPAR + 2 * dperp - f.dpar - f.dperp1;
+}
+
+__device__ inline void bingham_fan_setup(CDM_F64 fintra, CDM_F64 kappa, CDM_F64 beta, bingham_fan& f) {
+  CDM_F64 h[5];
+  for (int j = 0; j < 5; ++j) h[j] = bingham_fan_job(j, kappa, beta);
+  bingham_fan_combine(fintra, h, f);
 }
 
 // Gradient in the fan's local frame for measurement CFP (g, b): components

@@ -16,6 +16,7 @@
 
 #define CDM_VOXEL_CACHE 4
 #define CDM_VOXEL_CACHE_DEPENDS 0x0Eu       // fintra, kappa, beta
+#define CDM_VOXEL_CACHE_JOBS 5              // its five 1F1 evaluations
 #define CDM_NUMERICAL_MASK 0x7Cu            // kappa, beta, th, ph, psi
 #define CDM_MEAS_VOXEL_CACHE 1              // intra-axonal signal per measurement
 #define CDM_MEAS_VOXEL_CACHE_DEPENDS 0x7Cu  // kappa, beta, th, ph, psi
@@ -23,6 +24,19 @@
 MACRO void Voxel_Cache(int npar, T* P, T* FixP, CDM_F64* vc) {
   bingham_fan f;
   bingham_fan_setup((CDM_F64)P[1], (CDM_F64)P[2], (CDM_F64)P[3], f);
+  vc[0] = f.nc;
+  vc[1] = f.dpar;
+  vc[2] = f.dperp1;
+  vc[3] = f.dperp2;
+}
+
+MACRO CDM_F64 Voxel_Cache_Job(int job, int npar, T* P, T* FixP) {
+  return bingham_fan_job(job, (CDM_F64)P[2], (CDM_F64)P[3]);
+}
+
+MACRO void Voxel_Cache_Combine(int npar, T* P, T* FixP, const CDM_F64* jobs, CDM_F64* vc) {
+  bingham_fan f;
+  bingham_fan_combine((CDM_F64)P[1], jobs, f);
   vc[0] = f.nc;
   vc[1] = f.dpar;
   vc[2] = f.dperp1;
