@@ -1,13 +1,14 @@
-"""Autotuner of the launch shape: voxel-group size (lanes per voxel) per
-model, measurement count and method.
+"""Autotuner of the launch shape: voxel-group size (lanes per voxel) and, for
+LM and MCMC, register budget, per model, measurement count and method.
 
 For every method it times the kernels at each compiled group size (1, 4,
-16, 64 lanes) on a synthetic volume large enough to fill the GPU, with a
-short representative workload (a few LM iterations, a few MCMC jumps), and
-keeps the fastest (GridSearch keeps the built-in choice: its cost depends
-on the user's grid).  Results go to a text table, one line per choice:
+16, 64 lanes) and register budget (1, 2, 4 resident blocks per CU) on a
+synthetic volume large enough to fill the GPU, with a short representative
+workload (a few LM iterations, a few MCMC jumps), and keeps the fastest
+(GridSearch keeps the built-in choice: its cost depends on the user's grid).
+Results go to a text table, one line per choice:
 
-    <model> <nmeas> <method> <lanes> <ms>
+    <model> <nmeas> <method> <lanes> <ms> <blocks>
 
 which the C-ABI reads at model open (cdm_model_load_tuning) and this module
 applies.  Run: python -m cudimot_amd.autotune [--models A,B] [--voxels N]
@@ -24,6 +25,7 @@ TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.txt")
 METHOD_NAMES = {capi.GRIDSEARCH: "gridsearch", capi.LEVMAR: "levmar", capi.MCMC: "mcmc",
                 capi.PREDICTED: "predicted", capi.BIC_AIC: "bic_aic"}
 LANES = (1, 4, 16, 64)
+OCCUPANCY = (1, 2, 4)
 
 
 def _volume(model, m, n, torch):
@@ -58,17 +60,21 @@ def tune_model(model, n=65536, reps=2):
     res = {}
     for method, fn in runs.items():
         best = None
+        budgets = OCCUPANCY if method in (capi.LEVMAR, capi.MCMC) else (1,)
         for L in LANES:
             m.set_group(method, L)
-            fn()
-            torch.cuda.synchronize()
-            t0 = time.time()
-            for _ in range(reps):
+            for W in budgets:
+                if method in (capi.LEVMAR, capi.MCMC):
+                    m.set_occupancy(method, W)
                 fn()
-            torch.cuda.synchronize()
-            ms = (time.time() - t0) * 1000 / reps
-            if best is None or ms < best[1]:
-                best = (L, ms)
+                torch.cuda.synchronize()
+                t0 = time.time()
+                for _ in range(reps):
+                    fn()
+                torch.cuda.synchronize()
+                ms = (time.time() - t0) * 1000 / reps
+                if best is None or ms < best[1]:
+                    best = (L, ms, W)
         res[method] = best
     return cfp.shape[0], res
 
@@ -82,11 +88,11 @@ def write_table(rows, path=TABLE):
             if len(f) >= 4 and not line.startswith("#") and (f[0], int(f[1]), f[2]) not in new_keys:
                 keep.append(line.rstrip("\n"))
     with open(path, "w") as fh:
-        fh.write("# model nmeas method lanes ms  (written by python -m cudimot_amd.autotune)\n")
+        fh.write("# model nmeas method lanes ms blocks  (written by python -m cudimot_amd.autotune)\n")
         for line in keep:
             fh.write(line + "\n")
-        for model, nmeas, method, lanes, ms in rows:
-            fh.write("%s %d %s %d %.3f\n" % (model, nmeas, method, lanes, ms))
+        for model, nmeas, method, lanes, ms, occ in rows:
+            fh.write("%s %d %s %d %.3f %d\n" % (model, nmeas, method, lanes, ms, occ))
 
 
 def main(argv=None):
@@ -99,8 +105,8 @@ def main(argv=None):
     rows = []
     for model in models:
         nmeas, res = tune_model(model, a.voxels)
-        for method, (L, ms) in res.items():
-            rows.append((model, nmeas, METHOD_NAMES[method], L, ms))
+        for method, (L, ms, W) in res.items():
+            rows.append((model, nmeas, METHOD_NAMES[method], L, ms, W))
         print(model, nmeas, {METHOD_NAMES[k]: v for k, v in res.items()}, flush=True)
     write_table(rows, a.out)
     print("wrote", a.out)

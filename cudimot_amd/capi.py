@@ -8,7 +8,7 @@ import ctypes
 import os
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "lib", "libcudimot.so")
+LIB_PATH = os.environ.get("CDM_LIBRARY") or os.path.join(PKG, "lib", "libcudimot.so")
 
 GRIDSEARCH, LEVMAR, MCMC, PREDICTED, BIC_AIC = range(5)
 
@@ -51,6 +51,8 @@ def lib():
             "cdm_model_set_group": (i, [vp, i, i]),
             "cdm_model_get_group": (i, [vp, i, i, i]),
             "cdm_model_load_tuning": (i, [vp, cp]),
+            "cdm_model_set_occupancy": (i, [vp, i, i]),
+            "cdm_model_get_occupancy": (i, [vp, i, i]),
             "cdm_model_set_mcmc_blocks": (i, [vp, i, i]),
             "cdm_gridsearch": (i, [vp, i, i, vp, vp, vp, vp, vp, vp]),
             "cdm_levmar": (i, [vp, i, i, vp, vp, vp, vp, i, i, vp, vp, vp]),
@@ -167,6 +169,16 @@ class Model:
 
     def get_group(self, method, nmeas, nvox=0):
         r = lib().cdm_model_get_group(self.h, method, nmeas, int(nvox))
+        if r < 0:
+            raise CdmError(lib().cdm_last_error().decode())
+        return r
+
+    def set_occupancy(self, method, blocks):
+        """Register budget of the LM / MCMC kernels (1, 2, 4 blocks per CU; 0: automatic)."""
+        check(lib().cdm_model_set_occupancy(self.h, method, int(blocks)))
+
+    def get_occupancy(self, method, nmeas):
+        r = lib().cdm_model_get_occupancy(self.h, method, nmeas)
         if r < 0:
             raise CdmError(lib().cdm_last_error().decode())
         return r

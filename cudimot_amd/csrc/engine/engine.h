@@ -804,8 +804,11 @@ __device__ __forceinline__ void cdm_levmar_body(const cdm_lm_args& a, int marqua
   }
 }
 
-template <int L>
-__global__ void __launch_bounds__(cdm_block) cdm_levmar_kernel(cdm_lm_args a, int marquardt) {
+// W: resident 256-thread blocks per CU the register allocation must allow
+// (1, 2 or 4; the compiler caps registers and spills beyond them).  Chosen
+// per model and measurement count by the autotuner with L.
+template <int L, int W>
+__global__ void __launch_bounds__(cdm_block, W) cdm_levmar_kernel(cdm_lm_args a, int marquardt) {
   cdm_levmar_body<L>(a, marquardt, blockIdx.x);
 }
 
@@ -1182,8 +1185,8 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
   }
 }
 
-template <int L, bool RICIAN>
-__global__ void __launch_bounds__(cdm_block) cdm_mcmc_kernel(cdm_mcmc_args a) {
+template <int L, bool RICIAN, int W>
+__global__ void __launch_bounds__(cdm_block, W) cdm_mcmc_kernel(cdm_mcmc_args a) {
   cdm_mcmc_body<L, RICIAN>(a, blockIdx.x);
 }
 

@@ -12,6 +12,14 @@
     default: return hipErrorInvalidValue;  \
   }
 
+#define CDM_DISPATCH_W(W, CALL)            \
+  switch (W) {                             \
+    case 1: { constexpr int WG = 1; CALL; } break; \
+    case 2: { constexpr int WG = 2; CALL; } break; \
+    case 4: { constexpr int WG = 4; CALL; } break; \
+    default: return hipErrorInvalidValue;  \
+  }
+
 static hipError_t op_gridsearch(int L, const cdm::Geom& g, float* params, int ngp, const int* gparams, int ncombs,
                                 const float* grid_f, const double* grid_d, const int* comb_id, const cdm::Consts& c,
                                 int* kept, hipStream_t s) {
@@ -33,27 +41,27 @@ static hipError_t op_gridsearch(int L, const cdm::Geom& g, float* params, int ng
   return hipGetLastError();
 }
 
-static hipError_t op_levmar(int L, const cdm::Geom& g, float* params, int max_iters, int marquardt,
+static hipError_t op_levmar(int L, int W, const cdm::Geom& g, float* params, int max_iters, int marquardt,
                             const cdm::Consts& c, int* iters, float* cost, hipStream_t s) {
   if (g.nvox <= 0) return hipSuccess;
   if (g.cfp_ts > cdm_cfp_max || g.fixp_ts > cdm_fixp_max) return hipErrorInvalidValue;
   const cdm_lm_args a = op_lm_args(g, params, max_iters, c, iters, cost);
-  CDM_DISPATCH_L(L, hipLaunchKernelGGL(cdm_levmar_kernel<LG>, op_grid(g.nvox, LG), dim3(cdm_block), op_lds(g), s, a,
-                                       marquardt));
+  CDM_DISPATCH_L(L, CDM_DISPATCH_W(W, hipLaunchKernelGGL((cdm_levmar_kernel<LG, WG>), op_grid(g.nvox, LG),
+                                                         dim3(cdm_block), op_lds(g), s, a, marquardt)));
   return hipGetLastError();
 }
 
-static hipError_t op_mcmc(int L, int rician, const cdm::Geom& g, const cdm::McmcLaunch& m, const cdm::Consts& c,
-                          hipStream_t s) {
+static hipError_t op_mcmc(int L, int W, int rician, const cdm::Geom& g, const cdm::McmcLaunch& m,
+                          const cdm::Consts& c, hipStream_t s) {
   if (g.nvox <= 0) return hipSuccess;
   if (g.cfp_ts > cdm_cfp_max || g.fixp_ts > cdm_fixp_max) return hipErrorInvalidValue;
   const cdm_mcmc_args a = op_mcmc_args(g, m, c);
-  if (rician) {
-    CDM_DISPATCH_L(L, hipLaunchKernelGGL((cdm_mcmc_kernel<LG, true>), op_grid(g.nvox, LG), dim3(cdm_block),
+  if (rician) {  // the Rician likelihood is compiled at the default register budget only
+    CDM_DISPATCH_L(L, hipLaunchKernelGGL((cdm_mcmc_kernel<LG, true, 1>), op_grid(g.nvox, LG), dim3(cdm_block),
                                          op_lds(g), s, a));
   } else {
-    CDM_DISPATCH_L(L, hipLaunchKernelGGL((cdm_mcmc_kernel<LG, false>), op_grid(g.nvox, LG), dim3(cdm_block),
-                                         op_lds(g), s, a));
+    CDM_DISPATCH_L(L, CDM_DISPATCH_W(W, hipLaunchKernelGGL((cdm_mcmc_kernel<LG, false, WG>), op_grid(g.nvox, LG),
+                                                           dim3(cdm_block), op_lds(g), s, a)));
   }
   return hipGetLastError();
 }
@@ -97,3 +105,4 @@ static hipError_t op_bic_aic(int L, int rician, const cdm::Geom& g, int nsamples
 }
 
 #undef CDM_DISPATCH_L
+#undef CDM_DISPATCH_W

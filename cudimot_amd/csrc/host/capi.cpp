@@ -22,10 +22,12 @@ struct cdm_model {
   const cdm::ModelOps* ops = nullptr;
   cdm::ModelSpec spec;
   int group[5] = {0, 0, 0, 0, 0};
-  // autotuned choices for this model: (measurement count, method, lanes)
+  int occ[5] = {0, 0, 0, 0, 0};  // register budget (resident blocks per CU); 0: automatic
+  // autotuned choices for this model: (measurement count, method, lanes, budget)
   struct Tuned {
     int nmeas, method, lanes;
     double ms;  // measured time of the tuning run (0 if not given)
+    int occ;    // register budget (1 if not given)
   };
   std::vector<Tuned> tuned;
   int mcmc_block = 0;
@@ -106,22 +108,38 @@ int default_group(int method, int nmeas) {
 // work serial.
 constexpr long long kLaneSlots = 256LL * 4 * 64 * 2;
 
-int group_for(const cdm_model* m, int method, int nmeas, int nvox) {
-  if (m->group[method]) return m->group[method];
-  int best = 0;
+// Tuned entry of the nearest measurement count (log ratio), or null.
+const cdm_model::Tuned* tuned_for(const cdm_model* m, int method, int nmeas) {
+  const cdm_model::Tuned* best = nullptr;
   double dist = 0;
   for (const auto& t : m->tuned) {
     if (t.method != method) continue;
     const double d = std::fabs(std::log((double)t.nmeas / (double)std::max(nmeas, 1)));
     if (!best || d < dist) {
-      best = t.lanes;
+      best = &t;
       dist = d;
     }
   }
-  int L = best ? best : default_group(method, nmeas);
+  return best;
+}
+
+int group_for(const cdm_model* m, int method, int nmeas, int nvox) {
+  if (m->group[method]) return m->group[method];
+  const cdm_model::Tuned* t = tuned_for(m, method, nmeas);
+  int L = t ? t->lanes : default_group(method, nmeas);
   while (nvox > 0 && L < 64 && (long long)nvox * L < kLaneSlots) L *= 4;
   return L;
 }
+
+// Register budget of the LM and MCMC kernels: explicit, else tuned, else 1
+// (the compiler's own allocation).
+int occ_for(const cdm_model* m, int method, int nmeas) {
+  if (m->occ[method]) return m->occ[method];
+  const cdm_model::Tuned* t = tuned_for(m, method, nmeas);
+  return t ? t->occ : 1;
+}
+
+bool valid_occ(int w) { return w == 1 || w == 2 || w == 4; }
 
 const char* const kMethodNames[5] = {"gridsearch", "levmar", "mcmc", "predicted", "bic_aic"};
 
@@ -158,8 +176,14 @@ int read_tuning(cdm_model* m, const std::string& path, std::string* err) {
       *err = path + ":" + std::to_string(lineno) + ": unknown method '" + method + "'";
       return -1;
     }
+    int occ = 1;
     if (!(ss >> ms)) ms = 0;
-    got.push_back({nmeas, k, lanes, ms});
+    else if (!(ss >> occ)) occ = 1;
+    if (!valid_occ(occ)) {
+      *err = path + ":" + std::to_string(lineno) + ": register budget must be 1, 2 or 4";
+      return -1;
+    }
+    got.push_back({nmeas, k, lanes, ms, occ});
   }
   m->tuned = got;
   return (int)got.size();
@@ -407,6 +431,19 @@ int cdm_model_set_group(cdm_model* m, int method, int lanes) {
   return 0;
 }
 
+int cdm_model_set_occupancy(cdm_model* m, int method, int blocks) {
+  if (!m) return fail("null model handle");
+  if (method != cdm::kLevMar && method != cdm::kMcmc) return fail("register budgets apply to LM and MCMC");
+  if (blocks != 0 && !valid_occ(blocks)) return fail("register budget must be 1, 2 or 4 blocks per CU (0: automatic)");
+  m->occ[method] = blocks;
+  return 0;
+}
+
+int cdm_model_get_occupancy(const cdm_model* m, int method, int nmeas) {
+  if (!m || method < 0 || method > 4) return fail("bad arguments");
+  return occ_for(m, method, nmeas);
+}
+
 int cdm_model_load_tuning(cdm_model* m, const char* path) {
   if (!m) return fail("null model handle");
   if (!path) {
@@ -454,7 +491,8 @@ int cdm_levmar(cdm_model* m, int nvox, int nmeas, const float* meas, float* para
   if (max_iters < 0) return fail("--iterLevMar must be >= 0");
   if (nvox == 0) return 0;
   const cdm::Geom g = make_geom(m, nvox, nmeas, meas, cfp, fixp);
-  const hipError_t e = m->ops->levmar(group_for(m, cdm::kLevMar, nmeas, nvox), g, params, max_iters, marquardt ? 1 : 0,
+  const hipError_t e = m->ops->levmar(group_for(m, cdm::kLevMar, nmeas, nvox), occ_for(m, cdm::kLevMar, nmeas), g,
+                                      params, max_iters, marquardt ? 1 : 0,
                                       make_consts(m), iters, cost, (hipStream_t)stream);
   return e == hipSuccess ? 0 : hip_fail(e, "Levenberg-Marquardt launch");
 }
@@ -479,6 +517,7 @@ int cdm_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params
   const cdm::Geom g = make_geom(m, nvox, nmeas, meas, cfp, fixp);
   const cdm::Consts c = make_consts(m);
   const int L = group_for(m, cdm::kMcmc, nmeas, nvox);
+  const int W = occ_for(m, cdm::kMcmc, nmeas);
   hipStream_t s = (hipStream_t)stream;
 
   cdm::McmcLaunch ml;
@@ -503,7 +542,7 @@ int cdm_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params
       for (int b = 0; b < len; b += blk) {
         ml.begin = b;
         ml.count = std::min(blk, len - b);
-        const hipError_t e = m->ops->mcmc(L, rician ? 1 : 0, g, ml, c, q);
+        const hipError_t e = m->ops->mcmc(L, W, rician ? 1 : 0, g, ml, c, q);
         if (e != hipSuccess) return e;
       }
       return hipSuccess;

@@ -56,9 +56,10 @@ struct ModelOps {
   hipError_t (*gridsearch)(int L, const Geom& g, float* params, int ngp, const int* gparams, int ncombs,
                            const float* grid_f, const double* grid_d, const int* comb_id, const Consts& c,
                            int* kept, hipStream_t s);
-  hipError_t (*levmar)(int L, const Geom& g, float* params, int max_iters, int marquardt, const Consts& c,
+  // W: register budget, in resident 256-thread blocks per CU (1, 2 or 4)
+  hipError_t (*levmar)(int L, int W, const Geom& g, float* params, int max_iters, int marquardt, const Consts& c,
                        int* iters, float* cost, hipStream_t s);
-  hipError_t (*mcmc)(int L, int rician, const Geom& g, const McmcLaunch& m, const Consts& c, hipStream_t s);
+  hipError_t (*mcmc)(int L, int W, int rician, const Geom& g, const McmcLaunch& m, const Consts& c, hipStream_t s);
   hipError_t (*predicted)(int L, const Geom& g, int nsamples, const float* samples, float* out, hipStream_t s);
   hipError_t (*bic_aic)(int L, int rician, const Geom& g, int nsamples, const float* samples,
                         const float* tau_samples, float* bic, float* aic, hipStream_t s);

@@ -74,9 +74,16 @@ int cdm_model_grid(const cdm_model* m, int* grid_params, double* grid);
 int cdm_model_set_group(cdm_model* m, int method, int lanes);
 int cdm_model_get_group(const cdm_model* m, int method, int nmeas, int nvox);
 
+/* Register budget of the LM and MCMC kernels, in resident 256-thread blocks
+ * per CU the compiled variant allows for (1, 2 or 4: the compiler caps
+ * registers, spilling beyond them, for more waves in flight); 0 restores the
+ * automatic choice (tuned, else 1).  The Rician MCMC kernel has budget 1. */
+int cdm_model_set_occupancy(cdm_model* m, int method, int blocks);
+int cdm_model_get_occupancy(const cdm_model* m, int method, int nmeas);
+
 /* Autotuned launch table (no counterpart in the reference, whose launch
  * shape is fixed: one warp per voxel, 64 voxels per block, MCMC.cu:780-784).
- * Lines "<model> <nmeas> <method> <lanes> [ms]", method one of gridsearch,
+ * Lines "<model> <nmeas> <method> <lanes> [ms [blocks]]", method one of gridsearch,
  * levmar, mcmc, predicted, bic_aic; entries of other models are skipped.
  * A method without an explicit set_group uses the entry of the nearest
  * measurement count.  cdm_model_open loads <libdir>/../tuning.txt when it

@@ -110,3 +110,14 @@ def test_tuning_table_nearest_measurement_count(tmp_path):
     with pytest.raises(capi.CdmError, match="unknown method"):
         m.load_tuning(str(t))
     assert m.load_tuning(None) == 0
+    t.write_text("NODDI_Bingham 99 levmar 64 3.5 2\nNODDI_Bingham 99 mcmc 4 2.0\n")
+    assert m.load_tuning(str(t)) == 2
+    assert m.get_occupancy(capi.LEVMAR, 99) == 2
+    assert m.get_occupancy(capi.MCMC, 99) == 1
+    m.set_occupancy(capi.MCMC, 4)
+    assert m.get_occupancy(capi.MCMC, 99) == 4
+    with pytest.raises(capi.CdmError, match="register budget"):
+        m.set_occupancy(capi.MCMC, 3)
+    t.write_text("NODDI_Bingham 99 levmar 64 3.5 8\n")
+    with pytest.raises(capi.CdmError, match="register budget"):
+        m.load_tuning(str(t))
