@@ -29,6 +29,10 @@ static cdm_geom op_geom(const cdm::Geom& g) {
   return k;
 }
 
+static size_t op_lds_mcmc(const cdm::Geom& g, int L) {
+  return cdm_vslot_offset(g.nmeas, g.cfp_ts) + (size_t)(cdm_block / L) * sizeof(cdm_vslot);
+}
+
 static size_t op_lds(const cdm::Geom& g) {
   return (size_t)cdm_mc_offset(g.nmeas, g.cfp_ts) * sizeof(float) + (size_t)g.nmeas * cdm_mc * sizeof(CDM_F64);
 }

@@ -830,6 +830,22 @@ struct cdm_chain {  // FP64 chain state per voxel
   int tau_acc, tau_rej;
 };
 
+// Per-parameter chain state of one voxel while a launch runs, kept in LDS
+// (one slot per voxel group of the block) rather than registers: a step
+// indexes it by the proposed parameter, and registers are the kernel's
+// occupancy limit.  All lanes of the group write identical values.
+struct cdm_vslot {
+  double p[cdm_np], prior[cdm_np], sd[cdm_np];
+  int nacc[cdm_np], nrej[cdm_np];
+};
+
+// Byte offset of the slots in the dynamic LDS (after the CFP table and the
+// measurement cache, 16-byte aligned).
+__host__ __device__ __forceinline__ size_t cdm_vslot_offset(int nmeas, int cfp_ts) {
+  const size_t b = (size_t)cdm_mc_offset(nmeas, cfp_ts) * sizeof(float) + (size_t)nmeas * cdm_mc * sizeof(CDM_F64);
+  return (b + 15) / 16 * 16;
+}
+
 struct cdm_mcmc_args {
   cdm_geom g;
   float* params;         // [nvox][NP] in/out
@@ -982,8 +998,14 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
   const uint64_t gvox = (uint64_t)(a.voxel_offset + v);
   cdm_chain* st = &a.state[v];
 
-  double p[cdm_np], prior[cdm_np], sd[cdm_np];
-  int nacc[cdm_np], nrej[cdm_np];
+  cdm_vslot* vs = reinterpret_cast<cdm_vslot*>(reinterpret_cast<char*>(s_cfp) +
+                                               cdm_vslot_offset(a.g.nmeas, a.g.cfp_ts)) +
+                  threadIdx.x / L;
+  double* p = vs->p;
+  double* prior = vs->prior;
+  double* sd = vs->sd;
+  int* nacc = vs->nacc;
+  int* nrej = vs->nrej;
   double energy, tau = 0.0, tau_sd = 0.0;
   int tau_acc = 0, tau_rej = 0;
 
@@ -1107,7 +1129,11 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
         const bool mdep = (cdm_mvc_dep >> i) & 1u;
         CDM_F64 vc_new[cdm_vc];
         if (dep) cdm_prepare_dg<L>(p, fx, vc_new, lane);
-        const CDM_F64* vcu = dep ? vc_new : vc_cur;
+        // element-wise select: a pointer select between two register arrays
+        // would place both in scratch memory
+        CDM_F64 vcu[cdm_vc];
+#pragma unroll
+        for (int k = 0; k < cdm_vc; ++k) vcu[k] = dep ? vc_new[k] : vc_cur[k];
         double like;
         if constexpr (cdm_mvc > 0) {
           const size_t rs = (size_t)a.g.nmeas * cdm_mvc;
@@ -1127,10 +1153,8 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
         if (exp(energy - e_new) > d.uniform) {
           ++nacc[i];
           energy = e_new;
-          if (dep) {
 #pragma unroll
-            for (int k = 0; k < cdm_vc; ++k) vc_cur[k] = vc_new[k];
-          }
+          for (int k = 0; k < cdm_vc; ++k) vc_cur[k] = vcu[k];
           if (mdep) cur = 1 - cur;
         } else {
           ++nrej[i];

@@ -58,10 +58,10 @@ static hipError_t op_mcmc(int L, int W, int rician, const cdm::Geom& g, const cd
   const cdm_mcmc_args a = op_mcmc_args(g, m, c);
   if (rician) {  // the Rician likelihood is compiled at the default register budget only
     CDM_DISPATCH_L(L, hipLaunchKernelGGL((cdm_mcmc_kernel<LG, true, 1>), op_grid(g.nvox, LG), dim3(cdm_block),
-                                         op_lds(g), s, a));
+                                         op_lds_mcmc(g, LG), s, a));
   } else {
     CDM_DISPATCH_L(L, CDM_DISPATCH_W(W, hipLaunchKernelGGL((cdm_mcmc_kernel<LG, false, WG>), op_grid(g.nvox, LG),
-                                                           dim3(cdm_block), op_lds(g), s, a)));
+                                                           dim3(cdm_block), op_lds_mcmc(g, LG), s, a)));
   }
   return hipGetLastError();
 }
