@@ -250,7 +250,40 @@ def build(models=None, extra_dirs=(), force=False, jobs=None, batches=None):
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-ldl"])
     os.replace(tmp, LIB)
+    build_cli()
     return LIB
+
+
+BINDIR = os.path.join(PKG, "bin")
+CLI = os.path.join(BINDIR, "cudimot")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+
+def build_cli():
+    """Host driver ``bin/cudimot`` (csrc/cli): the reference's split_parts,
+    fit and merge_parts programs over the C-ABI.  Links the engine library
+    and zlib; symlinks named split_parts_<Model>, <Model> and
+    merge_parts_<Model> select the step as the reference's binaries do."""
+    os.makedirs(BINDIR, exist_ok=True)
+    inc = os.path.join(PKG, "..", "include")
+    deps = [os.path.join(CSRC, "cli"), os.path.join(inc, "cudimot_amd.h")]
+    flags = ["-O2", "-std=c++17", "-I", CSRC, "-I", inc, "-I", os.path.join(ROCM, "include"), "-D__HIP_PLATFORM_AMD__"]
+    objs = []
+    for f in ("nifti_io.cpp", "parts.cpp", "options.cpp", "driver.cpp"):
+        src = os.path.join(CSRC, "cli", f)
+        obj = os.path.join(BUILD, "cli_" + f[:-4] + ".o")
+        stamp = obj + ".sha"
+        key = _deps_hash([src] + deps)
+        if not (os.path.exists(obj) and os.path.exists(stamp) and open(stamp).read() == key):
+            _run(["g++"] + flags + ["-c", src, "-o", obj])
+            with open(stamp, "w") as fh:
+                fh.write(key)
+        objs.append(obj)
+    tmp = CLI + ".tmp"
+    _run(["g++", "-o", tmp] + objs + ["-L", LIBDIR, "-lcudimot", "-L", os.path.join(ROCM, "lib"), "-lamdhip64", "-lz",
+                                      "-Wl,-rpath,$ORIGIN/../lib:" + os.path.join(ROCM, "lib")])
+    os.replace(tmp, CLI)
+    return CLI
 
 
 OPC_LIB = os.path.join(LIBDIR, "libcdm_opcount.so")

@@ -381,6 +381,15 @@ int cdm_model_info(const cdm_model* m, int* nparams, int* cfp_tsize, int* fixp_t
   return 0;
 }
 
+int cdm_model_sizes(const cdm_model* m, int* ncfp, int* cfp_sizes, int* nfixp, int* fixp_sizes) {
+  if (!m) return fail("null model handle");
+  if (ncfp) *ncfp = m->ops->ncfp;
+  if (nfixp) *nfixp = m->ops->nfixp;
+  for (int i = 0; cfp_sizes && i < m->ops->ncfp; ++i) cfp_sizes[i] = m->ops->cfp_sizes[i];
+  for (int i = 0; fixp_sizes && i < m->ops->nfixp; ++i) fixp_sizes[i] = m->ops->fixp_sizes[i];
+  return 0;
+}
+
 int cdm_model_spec(const cdm_model* m, int* bound_types, double* bmin, double* bmax, int* prior_types, double* pa,
                    double* pb, int* fixed, double* p_init, int* init_provided) {
   if (!m) return fail("null model handle");

@@ -49,6 +49,12 @@ void cdm_model_close(cdm_model* m);
 /* NPARAMS and the summed CFP / FixP row sizes (Model.cu:248-260). */
 int cdm_model_info(const cdm_model* m, int* nparams, int* cfp_tsize, int* fixp_tsize);
 
+/* Number and row sizes of the common fixed parameters (CFP, per
+ * measurement) and of the per-voxel fixed parameters (FixP): the plug-in's
+ * MODEL::CFP_size / FixP_size (template/modelparameters.h:17-21; read by
+ * Model.cu:248-260).  Size arrays may be NULL. */
+int cdm_model_sizes(const cdm_model* m, int* ncfp, int* cfp_sizes, int* nfixp, int* fixp_sizes);
+
 /* Parsed specification (arrays of nparams; any pointer may be NULL).
  * Bounds and prior arguments are the parsed doubles; the kernels use them
  * rounded to float, as the reference's __constant__ float copies
