@@ -240,6 +240,8 @@ def build(models=None, extra_dirs=(), force=False, jobs=None, batches=None):
                       host_deps + [os.path.join(CSRC, "mlp"), os.path.join(CSRC, "engine", "philox.h")]))
     jobs_list.append((os.path.join(CSRC, "init", "orientation.hip"), os.path.join(BUILD, "orientation.o"),
                       host_deps + [os.path.join(CSRC, "init")]))
+    jobs_list.append((os.path.join(CSRC, "finish", "derived.hip"), os.path.join(BUILD, "derived.o"),
+                      host_deps + [os.path.join(CSRC, "finish")]))
     if force:
         for _, o, _ in jobs_list:
             if os.path.exists(o + ".sha"):

@@ -61,6 +61,10 @@ def lib():
             "cdm_bic_aic": (i, [vp, i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp]),
             "cdm_rician_clean": (i, [i, i, vp]),
             "cdm_tensor_frame": (i, [i, vp, i, i, vp, vp]),
+            "cdm_model_sizes": (i, [vp, ip, ip, ip, ip]),
+            "cdm_sample_mean": (i, [i, i, i, vp, vp, vp]),
+            "cdm_dyads": (i, [i, i, i, vp, i, i, i, vp, vp, vp]),
+            "cdm_noddi_maps": (i, [i, i, i, vp, i, i, vp, vp]),
             "cdm_batch_units": (i, [cp, i]),
             "cdm_batch_levmar": (i, [i, vp, i, i, vp]),
             "cdm_batch_mcmc": (i, [i, vp, i, i, i, i, i, u64, i, i, vp]),
@@ -274,6 +278,26 @@ def tensor_frame(dti_params, out, first=1, stream=None):
     """theta, phi, psi and eigenvalues [n][6] from tensor parameters [n][stride]."""
     n, stride = dti_params.shape
     check(lib().cdm_tensor_frame(n, ptr(dti_params), stride, first, ptr(out), _stream(stream)))
+
+
+def sample_mean(samples, out, stream=None):
+    """Mean over samples: samples [n][P][S] -> out [n][P] (fslmaths -Tmean)."""
+    n, P, S = samples.shape
+    check(lib().cdm_sample_mean(n, P, S, ptr(samples), ptr(out), _stream(stream)))
+
+
+def dyads(samples, theta, phi, out, dispersion=None, psi=-1, stream=None):
+    """Mean orientation [n][3] (and dispersion [n]) of the sampled directions
+    (theta, phi), or of the Bingham fanning directions when psi >= 0."""
+    n, P, S = samples.shape
+    check(lib().cdm_dyads(n, P, S, ptr(samples), theta, phi, psi, ptr(out), ptr(dispersion), _stream(stream)))
+
+
+def noddi_maps(samples, kappa, out, beta=-1, stream=None):
+    """Watson (beta < 0): OD [n][1].  Bingham: [n][5] = mean k2, ODIp, ODIs,
+    ODItot, DA."""
+    n, P, S = samples.shape
+    check(lib().cdm_noddi_maps(n, P, S, ptr(samples), kappa, beta, ptr(out), _stream(stream)))
 
 
 def sync(stream=None):

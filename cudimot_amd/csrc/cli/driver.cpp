@@ -455,6 +455,168 @@ void merge_parts(const Options& o) {
   }
 }
 
+// --------------------------------------------------------------- finish
+// The <Model>_finish.sh steps (mymodels/<Model>/<Model>_finish.sh): rename
+// the merged Param_k_samples volumes, then write sample means, mean fibre
+// and fanning orientations and the NODDI dispersion maps, computed on the
+// device by cdm_sample_mean / cdm_dyads / cdm_noddi_maps.
+struct FinishSpec {
+  const char* model;
+  std::vector<const char*> names;  // Param_k_samples -> <name>_samples (nullptr: keep)
+  std::vector<int> means;          // mean_<name>
+  struct Dir {
+    int th, ph, psi;
+    const char* out;
+  };
+  std::vector<Dir> dirs;  // psi < 0: make_dyadic_vectors (+ _dispersion); else getFanningOrientation
+  struct Noddi {
+    int kappa, beta;
+    const char* sfx;
+  };
+  std::vector<Noddi> noddi;
+  std::vector<int> fsum;     // mean_fsum: sum of these means (Ball_2_Sticks_finish.sh:45-47)
+  bool tensor = false;       // DiffusionTensor: Param_1..6 -> their means, and Tensor (DiffusionTensor_finish.sh:22-31)
+};
+
+const std::vector<FinishSpec>& finish_specs() {
+  static const std::vector<FinishSpec> specs = {
+      // Ball_1_Stick_finish.sh:21-31
+      {"Ball_1_Stick", {"S0", "d", "f1", "th1", "ph1"}, {0, 1, 2}, {{3, 4, -1, "dyads1"}}, {}, {}},
+      // Ball_2_Sticks_finish.sh:22-47 (maskdyads thresholds not reproduced)
+      {"Ball_2_Sticks", {"S0", "d", "f1", "th1", "ph1", "f2", "th2", "ph2"}, {0, 1, 2, 5},
+       {{3, 4, -1, "dyads1"}, {6, 7, -1, "dyads2"}}, {}, {2, 5}},
+      // no reference script: by analogy with Ball_2_Sticks
+      {"Ball_3_Sticks", {"S0", "d", "f1", "th1", "ph1", "f2", "th2", "ph2", "f3", "th3", "ph3"}, {0, 1, 2, 5, 8},
+       {{3, 4, -1, "dyads1"}, {6, 7, -1, "dyads2"}, {9, 10, -1, "dyads3"}}, {}, {2, 5, 8}},
+      // NODDI_Watson_finish.sh:22-35
+      {"NODDI_Watson", {"fiso", "fintra", "kappa", "th", "ph"}, {0, 1, 2}, {{3, 4, -1, "dyads1"}}, {{2, -1, ""}}, {}},
+      // NODDI_Bingham_finish.sh:24-56
+      {"NODDI_Bingham", {"fiso", "fintra", "kappa", "beta", "th", "ph", "psi"}, {0, 1, 2, 3},
+       {{4, 5, -1, "dyads1"}, {4, 5, 6, "fanningDir"}}, {{2, 3, ""}}, {}},
+      // NODDI_Bingham_2Fans_finish.sh:23-80
+      {"NODDI_Bingham_2Fans",
+       {"fiso", "fFan2", "fintra_f1", "kappa_f1", "beta_f1", "th_f1", "ph_f1", "psi_f1", "fintra_f2", "kappa_f2",
+        "beta_f2", "th_f2", "ph_f2", "psi_f2"},
+       {0, 1, 2, 3, 4, 8, 9, 10},
+       {{5, 6, -1, "dyads1"}, {11, 12, -1, "dyads2"}, {5, 6, 7, "fanningDir_f1"}, {11, 12, 13, "fanningDir_f2"}},
+       {{3, 4, "_f1"}, {9, 10, "_f2"}}, {}},
+      // DiffusionTensor_finish.sh:20-32 (-tensor_decomp not reproduced)
+      {"DiffusionTensor", {"S0", nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}, {0}, {}, {}, {}, true},
+  };
+  return specs;
+}
+
+std::string nifti_name(const std::string& dir, const std::string& base) {
+  const std::string f = find_volume(dir + "/" + base);
+  if (f.empty()) die("cannot find " + dir + "/" + base + " (.nii or .nii.gz)");
+  return f;
+}
+
+void finish(const std::string& model, const std::string& dir) {
+  const FinishSpec* spec = nullptr;
+  for (const FinishSpec& s : finish_specs())
+    if (model == s.model) spec = &s;
+  if (!spec) die("no finish step for model " + model);
+  Volume mask;
+  std::string e = read_volume(dir + "/nodif_brain_mask", mask);
+  if (!e.empty()) die(e);
+  const int P = (int)spec->names.size();
+  int nvox = 0, ns = 0;
+  std::vector<float> samples;  // [nvox][P][ns]
+  for (int k = 0; k < P; ++k) {
+    Volume v;
+    e = read_volume(dir + "/Param_" + std::to_string(k) + "_samples", v);
+    if (!e.empty()) die(e);
+    std::vector<double> m;
+    int nv = 0;
+    e = masked_matrix(v, mask, m, nv);
+    if (!e.empty()) die(e);
+    if (k == 0) {
+      nvox = nv;
+      ns = v.nt;
+      samples.assign((size_t)nvox * P * ns, 0.0f);
+    } else if (v.nt != ns) {
+      die("Param_" + std::to_string(k) + "_samples has " + std::to_string(v.nt) + " samples, expected " +
+          std::to_string(ns));
+    }
+    for (int i = 0; i < nvox; ++i)
+      for (int s = 0; s < ns; ++s) samples[((size_t)i * P + k) * ns + s] = (float)m[(size_t)i * ns + s];
+  }
+  for (int k = 0; k < P; ++k)
+    if (spec->names[k]) {
+      const std::string from = nifti_name(dir, "Param_" + std::to_string(k) + "_samples");
+      const std::string ext = from.substr(from.find_last_of('/') + 1 + std::string("Param_" + std::to_string(k) + "_samples").size());
+      fs::rename(from, dir + "/" + spec->names[k] + "_samples" + ext);
+    }
+  Dev<float> d_s(samples.size()), d_mean((size_t)nvox * P);
+  d_s.up(samples);
+  check(cdm_sample_mean(nvox, P, ns, d_s.p, d_mean.p, nullptr), "sample means");
+  std::vector<float> mean((size_t)nvox * P);
+  check(cdm_sync(nullptr), "sample means");
+  d_mean.down(mean);
+  auto column = [&](const std::vector<float>& src, int stride, int col, int width) {
+    std::vector<double> out((size_t)nvox * width);
+    for (int i = 0; i < nvox; ++i)
+      for (int w = 0; w < width; ++w) out[(size_t)i * width + w] = src[(size_t)i * stride + col + w];
+    return out;
+  };
+  auto save = [&](const std::string& name, const std::vector<double>& vals, int nt) {
+    const std::string err = write_masked(dir + "/" + name, mask, vals, nvox, nt);
+    if (!err.empty()) die(err);
+  };
+  for (int k : spec->means) save(std::string("mean_") + spec->names[k], column(mean, P, k, 1), 1);
+  if (!spec->fsum.empty()) {
+    std::vector<double> f(nvox, 0.0);
+    for (int k : spec->fsum)
+      for (int i = 0; i < nvox; ++i) f[i] += mean[(size_t)i * P + k];
+    save("mean_fsum", f, 1);
+  }
+  if (spec->tensor) {
+    std::vector<double> t((size_t)nvox * 6);
+    for (int k = 1; k <= 6; ++k) {
+      save("Param_" + std::to_string(k) + "_samples", column(mean, P, k, 1), 1);
+      for (int i = 0; i < nvox; ++i) t[(size_t)i * 6 + k - 1] = mean[(size_t)i * P + k];
+    }
+    save("Tensor", t, 6);
+  }
+  for (const FinishSpec::Dir& d : spec->dirs) {
+    Dev<float> d_dy((size_t)nvox * 3), d_disp(nvox);
+    check(cdm_dyads(nvox, P, ns, d_s.p, d.th, d.ph, d.psi, d_dy.p, d.psi < 0 ? d_disp.p : nullptr, nullptr), "dyads");
+    std::vector<float> dy((size_t)nvox * 3), disp(nvox);
+    check(cdm_sync(nullptr), "dyads");
+    d_dy.down(dy);
+    save(d.out, column(dy, 3, 0, 3), 3);
+    if (d.psi < 0) {
+      d_disp.down(disp);
+      save(std::string(d.out) + "_dispersion", column(disp, 1, 0, 1), 1);
+    }
+  }
+  for (const FinishSpec::Noddi& nd : spec->noddi) {
+    const int w = nd.beta < 0 ? 1 : 5;
+    Dev<float> d_maps((size_t)nvox * w);
+    check(cdm_noddi_maps(nvox, P, ns, d_s.p, nd.kappa, nd.beta, d_maps.p, nullptr), "NODDI maps");
+    std::vector<float> maps((size_t)nvox * w);
+    check(cdm_sync(nullptr), "NODDI maps");
+    d_maps.down(maps);
+    const std::string sfx = nd.sfx;
+    if (nd.beta < 0) {
+      save("OD" + sfx, column(maps, 1, 0, 1), 1);
+      continue;
+    }
+    std::vector<double> k2((size_t)nvox * ns);  // the k2 = kappa - beta sample volume
+    for (int i = 0; i < nvox; ++i)
+      for (int s = 0; s < ns; ++s)
+        k2[(size_t)i * ns + s] = (float)(samples[((size_t)i * P + nd.kappa) * ns + s] -
+                                         samples[((size_t)i * P + nd.beta) * ns + s]);
+    save("k2" + sfx + "_samples", k2, ns);
+    save("mean_k2" + sfx, column(maps, 5, 0, 1), 1);
+    save("ODIp" + sfx, column(maps, 5, 1, 1), 1);
+    save("ODIs" + sfx, column(maps, 5, 2, 1), 1);
+    save("ODItot" + sfx, column(maps, 5, 3, 1), 1);
+    save("DA" + sfx, column(maps, 5, 4, 1), 1);
+  }
+}
+
 bool starts_with(const std::string& s, const std::string& p) { return s.compare(0, p.size(), p) == 0; }
 
 bool is_model(const std::string& name) {
@@ -482,17 +644,43 @@ int main(int argc, char** argv) {
   } else if (is_model(prog)) {
     step = "fit";
     model = prog;
+  } else if (prog.size() > 7 && prog.compare(prog.size() - 7, 7, "_finish") == 0) {
+    step = "finish";
+    model = prog.substr(0, prog.size() - 7);
   } else {
     if (argc < 2 || argv[1][0] == '-') {
-      std::cerr << "Usage: " << prog << " split|fit|merge --model=<plug-in> [options]\n\n" << usage(prog);
+      std::cerr << "Usage: " << prog << " split|fit|merge --model=<plug-in> [options]\n"
+                << "       " << prog << " finish --model=<plug-in> <output directory>\n\n"
+                << usage(prog);
       return 2;
     }
     step = argv[1];
     first = 2;
-    if (step != "split" && step != "fit" && step != "merge") {
-      std::cerr << "unknown step '" << step << "' (split, fit or merge)\n";
+    if (step != "split" && step != "fit" && step != "merge" && step != "finish") {
+      std::cerr << "unknown step '" << step << "' (split, fit, merge or finish)\n";
       return 2;
     }
+  }
+  if (step == "finish") {  // <Model>_finish.sh <directory>
+    std::string dir;
+    for (int a = first; a < argc; ++a) {
+      const std::string s = argv[a];
+      if (starts_with(s, "--model="))
+        model = s.substr(8);
+      else if (dir.empty() && s[0] != '-')
+        dir = s;
+      else {
+        std::cerr << "Usage: " << prog << " finish --model=<plug-in> <output directory>\n";
+        return 2;
+      }
+    }
+    if (dir.empty() || model.empty()) {
+      std::cerr << "Usage: " << prog << " finish --model=<plug-in> <output directory>\n"
+                << "expects the merged Param_k_samples volumes and nodif_brain_mask in the directory\n";
+      return 2;
+    }
+    finish(model, dir);
+    return 0;
   }
   Options o;
   const std::string err = parse_options(argc, argv, first, o);

@@ -252,6 +252,30 @@ int cdm_batch_mcmc(int nitems, const cdm_batch_item* items, int nburn, int njump
  * the Bingham frame, and the eigenvalues in descending order. */
 int cdm_tensor_frame(int n, const float* dti, int stride, int first, float* out, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Derived maps of the <Model>_finish.sh steps, from MCMC samples
+ * [nvox][nparams][nsamples] on the device (or parameters, nsamples = 1). */
+
+/* Mean of each parameter's samples: mean [nvox][nparams] (fslmaths -Tmean
+ * of each Param_k_samples volume, e.g. NODDI_Watson_finish.sh:30-32). */
+int cdm_sample_mean(int nvox, int nparams, int nsamples, const float* samples, float* mean, void* stream);
+
+/* Mean orientation: principal eigenvector of the mean dyadic tensor of the
+ * sampled directions, dyads [nvox][3] (sign: z >= 0), and dispersion
+ * [nvox] = 1 - its eigenvalue (nullable).  psi < 0: fibre directions
+ * (theta, phi) as FSL's make_dyadic_vectors (Ball_1_Stick_finish.sh:31);
+ * psi >= 0: Bingham fanning directions (utils/getFanningOrientation.cc:39-76). */
+int cdm_dyads(int nvox, int nparams, int nsamples, const float* samples, int theta, int phi, int psi, float* dyads,
+              float* dispersion, void* stream);
+
+/* NODDI dispersion maps from the kappa (and beta) samples.  beta < 0
+ * (Watson): maps [nvox][1] = OD = 2/pi atan(1/mean kappa)
+ * (NODDI_Watson_finish.sh:35).  beta >= 0 (Bingham, k2 = kappa - beta):
+ * maps [nvox][5] = mean k2, ODIp, ODIs, ODItot, DA
+ * (NODDI_Bingham_finish.sh:32-56). */
+int cdm_noddi_maps(int nvox, int nparams, int nsamples, const float* samples, int kappa, int beta, float* maps,
+                   void* stream);
+
 /* Host-side data preparation for --rician: per voxel, replace non-positive
  * signals by the smallest positive one when some are positive
  * (dMRI_Data.cu:14-31).  meas is HOST memory [nvox][nmeas]. */

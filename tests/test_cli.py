@@ -257,3 +257,49 @@ def test_split_fit_merge_equals_direct_calls(tmp_path, mcmc):
     for name, key in (("BIC", "bic"), ("AIC", "aic")):
         x, _ = nu.read_nifti(str(tmp_path / "out" / (name + ".nii.gz")))
         np.testing.assert_array_equal(nu.masked(x, mask)[:, 0], np.concatenate(want[key]))
+
+
+def test_finish_usage_errors(tmp_path):
+    r = subprocess.run([_cli(), "finish", "--model=NODDI_Bingham"], capture_output=True, text=True)
+    assert r.returncode == 2 and "output directory" in r.stderr
+    r = subprocess.run([_cli(), "finish", "--model=toymodel", str(tmp_path)], capture_output=True, text=True)
+    assert r.returncode == 255 and "no finish step" in r.stderr
+
+
+@pytest.mark.gpu
+def test_finish_noddi_bingham_maps_match_oracle(tmp_path):
+    from oracle import finish_ref as fr
+    from tests.test_finish import _stick_samples
+    shape = (6, 5, 4)
+    rng = np.random.default_rng(31)
+    mask = (rng.uniform(size=shape) > 0.3).astype(np.uint8)
+    nvox, S = int(mask.sum()), 20
+    x = _stick_samples(nvox, S, 32)  # [nvox][7][S]: fiso, fintra, kappa, beta, th, ph, psi
+    nu.write_nifti(str(tmp_path / "nodif_brain_mask.nii.gz"), mask)
+    sel = mask.reshape(-1, order="F") > 0
+    for k in range(7):
+        flat = np.zeros((int(np.prod(shape)), S), dtype=np.float32)
+        flat[sel] = x[:, k, :]
+        nu.write_nifti(str(tmp_path / ("Param_%d_samples.nii.gz" % k)), flat.reshape(shape + (S,), order="F"))
+    r = subprocess.run([_cli(), "finish", "--model=NODDI_Bingham", str(tmp_path)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+    def get(name):
+        v, _ = nu.read_nifti(str(tmp_path / (name + ".nii.gz")))
+        return nu.masked(v, mask)
+    names = ["fiso", "fintra", "kappa", "beta", "th", "ph", "psi"]
+    for k, n in enumerate(names):  # renamed, unchanged
+        np.testing.assert_array_equal(get(n + "_samples"), x[:, k, :])
+        assert not os.path.exists(tmp_path / ("Param_%d_samples.nii.gz" % k))
+    mean = fr.sample_mean(x)
+    for k in range(4):
+        np.testing.assert_allclose(get("mean_" + names[k])[:, 0], mean[:, k], rtol=1e-6)
+    np.testing.assert_array_equal(get("k2_samples"), (x[:, 2, :] - x[:, 3, :]).astype(np.float32))
+    maps = fr.noddi_maps(x, 2, 3)
+    for i, n in enumerate(["mean_k2", "ODIp", "ODIs", "ODItot", "DA"]):
+        np.testing.assert_allclose(get(n)[:, 0], maps[:, i], rtol=2e-6, atol=1e-7)
+    for name, psi in (("dyads1", None), ("fanningDir", 6)):
+        d, disp = fr.dyads(x, 4, 5, psi)
+        np.testing.assert_allclose(np.abs(np.sum(get(name) * d, axis=1)), 1.0, atol=2e-5)
+        if psi is None:
+            np.testing.assert_allclose(get("dyads1_dispersion")[:, 0], disp, atol=2e-6)
