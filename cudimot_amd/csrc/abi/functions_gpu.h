@@ -89,6 +89,19 @@ CDM_FUNC void tsincos(double x, double* s, double* c) { *s = sin(x); *c = cos(x)
 CDM_FUNC float terf(float x) { return erff(x); }
 CDM_FUNC double terf(double x) { return erf(x); }
 
+// Reciprocal for the iterates of a root finder (engine extension): on the
+// GPU the hardware estimate refined by one Newton-Raphson step, accurate to
+// a few ulp but not correctly rounded, so for values a following iteration
+// corrects rather than for results; plain division on the host.
+#if defined(__HIP_DEVICE_COMPILE__)
+CDM_FUNC double trcp_iter(double x) {
+  const double r = __builtin_amdgcn_rcp(x);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+#else
+CDM_FUNC double trcp_iter(double x) { return 1.0 / x; }
+#endif
+
 // Central-difference step for NUMERICAL(i) by precision (see
 // macro_numerical.h): the reference's constants for double, a wider relative
 // step and floor for float.

@@ -131,6 +131,7 @@ CDM_ABI_1(tlog, log)
 CDM_ABI_1(texp, exp)
 CDM_ABI_1(terf, erf)
 #undef CDM_ABI_1
+inline cdm_cd trcp_iter(cdm_cd x) { return 1.0 / x; }
 inline cdm_cd tan_gpu(cdm_cf x) { return cdm_count::tan(cdm_cd(x)); }
 inline cdm_cd tan_gpu(cdm_cd x) { return cdm_count::tan(x); }
 inline cdm_cd sqrt_gpu(cdm_cf x) { return cdm_count::sqrt(cdm_cd(x)); }
