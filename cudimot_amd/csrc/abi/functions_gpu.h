@@ -93,6 +93,15 @@ CDM_FUNC double terf(double x) { return erf(x); }
 // GPU the hardware estimate refined by one Newton-Raphson step, accurate to
 // a few ulp but not correctly rounded, so for values a following iteration
 // corrects rather than for results; plain division on the host.
+// Single-precision cosine for starting values (engine extension): the
+// hardware approximation on the GPU (absolute error ~1e-6 for arguments of
+// a few radians), cosf on the host.
+#if defined(__HIP_DEVICE_COMPILE__)
+CDM_FUNC float tcos_start(float x) { return __cosf(x); }
+#else
+CDM_FUNC float tcos_start(float x) { return cosf(x); }
+#endif
+
 #if defined(__HIP_DEVICE_COMPILE__)
 CDM_FUNC double trcp_iter(double x) {
   const double r = __builtin_amdgcn_rcp(x);
