@@ -20,6 +20,7 @@
 #define CDM_NUMERICAL_MASK 0x7Cu            // kappa, beta, th, ph, psi
 #define CDM_MEAS_VOXEL_CACHE 1              // intra-axonal signal per measurement
 #define CDM_MEAS_VOXEL_CACHE_DEPENDS 0x7Cu  // kappa, beta, th, ph, psi
+#define CDM_SIGNAL_PARTIALS 1               // signal and analytic entries together
 
 MACRO void Voxel_Cache(int npar, T* P, T* FixP, CDM_F64* vc) {
   bingham_fan f;
@@ -98,6 +99,18 @@ MACRO void Partial_Derivatives_Cached(int npar, T* P, T* CFP, T* FixP, const CDM
   const T iso = exp_gpu(-CFP[3] * (T)CDM_NODDI_DISO);
   derivatives[0] = FixP[0] * (iso - fan);
   derivatives[1] = FixP[0] * ((T)1 - P[0]) * (i - e);
+}
+
+// The signal of Predicted_Signal_Cached with the analytic entries of
+// Partial_Derivatives_Cached, from one evaluation of the fan's parts.
+MACRO T Predicted_Signal_Partials_Cached(int npar, T* P, T* CFP, T* FixP, const CDM_F64* vc, const CDM_F64* mc,
+                                         T* derivatives) {
+  T fan, e, i;
+  bingham1_parts(P, CFP, vc, fan, e, i);
+  const T iso = exp_gpu(-CFP[3] * (T)CDM_NODDI_DISO);
+  derivatives[0] = FixP[0] * (iso - fan);
+  derivatives[1] = FixP[0] * ((T)1 - P[0]) * (i - e);
+  return FixP[0] * (((T)1 - P[0]) * fan + P[0] * iso);
 }
 
 MACRO void Partial_Derivatives(int npar, T* P, T* CFP, T* FixP, T* derivatives) {

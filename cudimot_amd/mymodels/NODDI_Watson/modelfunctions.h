@@ -17,6 +17,7 @@
 #define CDM_VOXEL_CACHE_DEPENDS 0x06u  // fintra, kappa
 #define CDM_MEAS_CACHE 8               // L_0..L_6 (b Dpar), exp(-b Diso)
 #define CDM_NUMERICAL_MASK 0x1Cu       // kappa, th, ph
+#define CDM_SIGNAL_PARTIALS 1          // signal and analytic entries together
 
 MACRO void Voxel_Cache(int npar, T* P, T* FixP, CDM_F64* vc) {
   noddi_watson_hindered((CDM_F64)P[2], CDM_NODDI_DPAR * (1.0 - (CDM_F64)P[1]), vc[0], vc[1]);
@@ -68,6 +69,18 @@ MACRO void Partial_Derivatives_Cached(int npar, T* P, T* CFP, T* FixP, const CDM
   const T aniso = ((T)1 - P[1]) * extra + P[1] * intra;
   derivatives[0] = FixP[0] * ((T)mc[7] - aniso);
   derivatives[1] = FixP[0] * ((T)1 - P[0]) * (intra - extra);
+}
+
+// The signal of Predicted_Signal_Cached with the analytic entries of
+// Partial_Derivatives_Cached, from one evaluation of the parts.
+MACRO T Predicted_Signal_Partials_Cached(int npar, T* P, T* CFP, T* FixP, const CDM_F64* vc, const CDM_F64* mc,
+                                         T* derivatives) {
+  T extra, intra;
+  watson_parts(P, CFP, vc, mc, extra, intra);
+  const T aniso = ((T)1 - P[1]) * extra + P[1] * intra;
+  derivatives[0] = FixP[0] * ((T)mc[7] - aniso);
+  derivatives[1] = FixP[0] * ((T)1 - P[0]) * (intra - extra);
+  return FixP[0] * (((T)1 - P[0]) * aniso + P[0] * (T)mc[7]);
 }
 
 MACRO void Partial_Derivatives(int npar, T* P, T* CFP, T* FixP, T* derivatives) {

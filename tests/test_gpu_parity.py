@@ -5,9 +5,11 @@ Tolerances (stated per check):
   * forward signal (FP32 on GPU vs FP64): max |rel| <= 2e-5 of the voxel's S0 scale
   * GridSearch: kept combination index identical for every voxel
   * Levenberg-Marquardt: final cost within 1e-3 relative for >= 97% of voxels
-    (>= 95% for NODDI_Bingham, the rest unbiased), and S0/diffusivity/fraction
-    estimates within 1e-3 relative for >= 90%; multi-stick models (multi-
-    modal): GPU not worse than the oracle more often than better
+    (the rest unbiased), and S0/diffusivity/fraction estimates within 1e-3
+    relative for >= 90%; NODDI_Bingham (flat ridges): within 2% (4% for the
+    parameters) of the oracle's agreement with itself under one-ulp data
+    changes; multi-stick models (multi-modal): GPU not worse than the oracle
+    more often than better
   * MCMC: draw-for-draw identical samples (<= 1e-4 rel) for >= 95% of voxels
     over the first jumps; posterior means within 0.35 posterior SD (median
     over voxels) for long chains
@@ -158,19 +160,41 @@ def _near_truth(model, t, rng):
     return p
 
 
+_ORACLE_LM = {}
+
+
+def _oracle_levmar(model, start, n):
+    """Volume, start and the oracle's LM result, once per (model, start, n)
+    for all group sizes.  For NODDI_Bingham also the oracle's own spread:
+    LM again on the data with every value moved by about one FP32 ulp."""
+    key = (model, start, n)
+    if key not in _ORACLE_LM:
+        from cudimot_amd import capi
+        o = bind.OracleModel(model)
+        spec = capi.Model(model).spec()
+        cfp, t, fx, noisy = H.volume(o, model, n, seed=31)
+        p0 = _start(model, spec, noisy, cfp, t) if start == "init" else _near_truth(model, t, np.random.default_rng(4))
+        noisy32 = noisy.astype(np.float32).astype(np.float64)
+        cfp32 = cfp.astype(np.float32).astype(np.float64)
+        p0_32 = p0.astype(np.float32).astype(np.float64)
+        p_ref, _, c_ref = o.levmar(_spec_arrays(spec), noisy32, p0_32, cfp32, fx, max_iters=200)
+        spread = None
+        if model == "NODDI_Bingham":
+            ulp = 1.2e-7 * np.random.default_rng(5).choice([-1.0, 1.0], size=noisy32.shape)
+            pert = (noisy32 * (1 + ulp)).astype(np.float32).astype(np.float64)
+            p_s, _, _ = o.levmar(_spec_arrays(spec), pert, p0_32, cfp32, fx, max_iters=200)
+            c_s = ((o.signal(p_s, cfp32, fx) - noisy32) ** 2).sum(1)
+            spread = _agreement(model, p_s, p_ref, c_s, c_ref)
+        _ORACLE_LM[key] = (o, cfp, fx, noisy, p0, noisy32, cfp32, p_ref, c_ref, spread)
+    return _ORACLE_LM[key]
+
+
 def _run_levmar(torch_cuda, model, lanes, start, n=None):
     from cudimot_amd import capi
-    o = bind.OracleModel(model)
     m = capi.Model(model)
     m.set_group(capi.LEVMAR, lanes)
-    spec = m.spec()
-    n = n or {"Ball_2_Sticks": 256, "Ball_3_Sticks": 256, "NODDI_Watson": 256, "NODDI_Bingham": 192}.get(model, 1024)
-    cfp, t, fx, noisy = H.volume(o, model, n, seed=31)
-    p0 = _start(model, spec, noisy, cfp, t) if start == "init" else _near_truth(model, t, np.random.default_rng(4))
-    noisy32 = noisy.astype(np.float32).astype(np.float64)
-    cfp32 = cfp.astype(np.float32).astype(np.float64)
-    p0_32 = p0.astype(np.float32).astype(np.float64)
-    p_ref, it_ref, c_ref = o.levmar(_spec_arrays(spec), noisy32, p0_32, cfp32, fx, max_iters=200)
+    n = n or {"Ball_2_Sticks": 256, "Ball_3_Sticks": 256, "NODDI_Watson": 256, "NODDI_Bingham": 512}.get(model, 1024)
+    o, cfp, fx, noisy, p0, noisy32, cfp32, p_ref, c_ref, spread = _oracle_levmar(model, start, n)
     dp = H.dev(p0)
     cost = torch_cuda.zeros((n,), dtype=torch_cuda.float32, device="cuda")
     iters = torch_cuda.zeros((n,), dtype=torch_cuda.int32, device="cuda")
@@ -178,34 +202,47 @@ def _run_levmar(torch_cuda, model, lanes, start, n=None):
     capi.sync()
     p = H.host(dp)
     c_at = ((o.signal(p, cfp32, fx) - noisy32) ** 2).sum(1)  # cost at the GPU solution, FP64
-    return p, p_ref, c_at, c_ref, H.host(cost)
+    return p, p_ref, c_at, c_ref, H.host(cost), spread
 
 
 LM_IDX = {"Ball_1_Stick": [0, 1, 2], "Ball_2_Sticks": [0, 1], "Ball_3_Sticks": [0, 1],
           "DiffusionTensor": [0, 1, 4, 6], "toymodel": [0, 1], "NODDI_Watson": [1, 2], "NODDI_Bingham": [1, 2]}
 
 
+def _agreement(model, p, p_ref, c_at, c_ref):
+    """(cost within 1e-3, worse by > 1e-3, better by > 1e-3, parameters
+    LM_IDX within 1e-3): fractions of voxels."""
+    rel = np.abs(c_at - c_ref) / np.maximum(c_ref, 1e-30)
+    idx = LM_IDX[model]
+    prel = np.abs(p[:, idx] - p_ref[:, idx]) / np.maximum(np.abs(p_ref[:, idx]), 1e-12)
+    return (np.mean(rel <= 1e-3), np.mean(c_at > c_ref * (1 + 1e-3)), np.mean(c_at < c_ref * (1 - 1e-3)),
+            np.mean(np.all(prel <= 1e-3, axis=1)))
+
+
 @pytest.mark.parametrize("model", ["Ball_1_Stick", "DiffusionTensor", "toymodel", "NODDI_Watson", "NODDI_Bingham"])
 @pytest.mark.parametrize("lanes", [1, 4, 16, 64])
 def test_levmar_parity(torch_cuda, model, lanes):
     """Same basin: single-fibre models from P_init (S0 from b=0), NODDI models
-    from a start near the truth (the GridSearch -> LM pipeline)."""
+    from a start near the truth (the GridSearch -> LM pipeline).
+
+    Bingham-NODDI fans with beta -> 0 leave psi free: LM ends on some point
+    of a flat ridge, and the oracle itself lands elsewhere for 3-4% of the
+    voxels (parameters: ~8%) when the data move by one FP32 ulp.  There the
+    GPU (FP32 signals) must agree with the oracle as well as the oracle
+    agrees with itself, within 2% (parameters 4%), and not favour worse
+    minima."""
     start = "near" if model.startswith("NODDI") else "init"
-    p, p_ref, c_at, c_ref, c = _run_levmar(torch_cuda, model, lanes, start)
-    rel = np.abs(c_at - c_ref) / np.maximum(c_ref, 1e-30)
-    frac_cost = np.mean(rel <= 1e-3)
-    print(model, lanes, "cost parity", frac_cost, "GPU worse", np.mean(c_at > c_ref * (1 + 1e-3)),
-          "GPU better", np.mean(c_at < c_ref * (1 - 1e-3)))
-    # Bingham-NODDI: a few degenerate fans (beta -> 0 leaves psi free) may
-    # settle in another point of a flat ridge; they must not favour the oracle
-    need = 0.95 if model == "NODDI_Bingham" else 0.97
-    assert frac_cost >= need, (model, lanes, frac_cost, np.sort(rel)[-10:])
-    assert np.mean(c_at > c_ref * (1 + 1e-3)) <= np.mean(c_at < c_ref * (1 - 1e-3)) + 0.03
+    p, p_ref, c_at, c_ref, c, spread = _run_levmar(torch_cuda, model, lanes, start)
+    frac_cost, worse, better, frac_p = _agreement(model, p, p_ref, c_at, c_ref)
+    print(model, lanes, "cost parity", frac_cost, "GPU worse", worse, "GPU better", better, "params", frac_p,
+          "oracle self-spread", spread)
+    need_cost, need_p = 0.97, 0.90
+    if spread is not None:
+        need_cost, need_p = spread[0] - 0.02, spread[3] - 0.04
+    assert frac_cost >= need_cost, (model, lanes, frac_cost, need_cost)
+    assert worse <= better + 0.03, (worse, better)
     assert np.median(np.abs(c - c_at) / np.maximum(c_at, 1e-30)) < 1e-4
-    idx = LM_IDX[model]
-    prel = np.abs(p[:, idx] - p_ref[:, idx]) / np.maximum(np.abs(p_ref[:, idx]), 1e-12)
-    frac_p = np.mean(np.all(prel <= 1e-3, axis=1))
-    assert frac_p >= 0.90, (model, lanes, frac_p)
+    assert frac_p >= need_p, (model, lanes, frac_p, need_p)
 
 
 @pytest.mark.parametrize("model", ["Ball_2_Sticks", "Ball_3_Sticks"])
@@ -215,7 +252,7 @@ def test_levmar_multistick_unbiased(torch_cuda, model, start):
     the reference's th2 one), so FP32 vs FP64 rounding can pick another
     nearby minimum: the GPU must not land in worse minima more often than
     the FP64 oracle does, and most voxels must agree."""
-    p, p_ref, c_at, c_ref, c = _run_levmar(torch_cuda, model, 16, start)
+    p, p_ref, c_at, c_ref, c, _ = _run_levmar(torch_cuda, model, 16, start)
     lr = np.log(c_at / c_ref)
     worse, better = np.mean(lr > 1e-3), np.mean(lr < -1e-3)
     print(model, start, "GPU worse", worse, "better", better)
