@@ -201,6 +201,57 @@ def test_bingham_compartments_vs_quadrature():
             assert abs(s - quad) / quad < tol, (kappa, beta, m, s, quad)
 
 
+def _hyp_saddle_ld(l):
+    """Second-order saddle-point 1F1 for eigenvalues l of -X (Kume & Wood), the
+    saddle point by bisection in extended precision."""
+    l = np.asarray(l, dtype=np.longdouble)
+    lo, hi = l.min() - np.longdouble(2), l.min() - np.longdouble(0.25)
+    for _ in range(200):
+        mid = (lo + hi) / 2
+        if 0.5 * np.sum(1 / (l - mid)) > 1:
+            hi = mid
+        else:
+            lo = mid
+    t = (lo + hi) / 2
+    y = 1 / (l - t)
+    K2, K3, K4 = 0.5 * np.sum(y ** 2), np.sum(y ** 3), 3 * np.sum(y ** 4)
+    T2 = K4 / (8 * K2 ** 2) - 5 * K3 ** 2 / (24 * K2 ** 3)
+    return 0.25 * np.sqrt(2 * np.prod(y) / K2) * np.exp(T2 - t)
+
+
+def test_bingham_intra_vs_eigenvalues_and_bisection():
+    """The per-measurement 1F1 (characteristic polynomial, single-precision
+    start, three Halley steps) against the textbook route: eigenvalues of
+    B - b Dpar g g^T (LAPACK) and the saddle point by extended-precision
+    bisection, over random and degenerate cases (beta = kappa, beta = 0,
+    gradients along the fibre, fanning or third axis: double roots)."""
+    o = bind.OracleModel("NODDI_Bingham")
+    rng = np.random.default_rng(17)
+    cases = []
+    for i in range(300):
+        kappa = rng.uniform(0.2, 64.0)
+        beta = [kappa * rng.uniform(0, 1), kappa, 0.0][i % 3]
+        cases.append((kappa, beta, rng.uniform(0, np.pi), rng.uniform(-np.pi, np.pi), rng.uniform(0, np.pi)))
+    worst = 0.0
+    for kappa, beta, th, ph, psi in cases:
+        st, ct, sp, cp, ss, cs = np.sin(th), np.cos(th), np.sin(ph), np.cos(ph), np.sin(psi), np.cos(psi)
+        n = np.array([st * cp, st * sp, ct])
+        f = np.array([-cs * sp - ct * cp * ss, cp * cs - ct * sp * ss, st * ss])
+        x = np.cross(f, n)
+        gs = [rng.normal(size=3) for _ in range(3)] + [n, f, x]
+        cfp = np.array([np.append(g / np.linalg.norm(g), b) for g in gs for b in (700.0, 2000.0, 3000.0)])
+        s = o.signal(np.array([[0.0, 1.0, kappa, beta, th, ph, psi]]), cfp, np.array([[1.0]]))[0]
+        B = kappa * np.outer(n, n) + beta * np.outer(f, f)
+        norm = _hyp_saddle_ld(-np.array([0.0, beta, kappa]))
+        for k, row in enumerate(cfp):
+            g = row[:3]
+            lam = np.linalg.eigvalsh(row[3] * 1.7e-3 * np.outer(g, g) - B)
+            want = float(_hyp_saddle_ld(lam) / norm)
+            worst = max(worst, abs(s[k] - want) / want)
+    print("worst relative error", worst)
+    assert worst < 1e-10, worst
+
+
 def test_noddi_b0_returns_s0():
     cfp = H.ACQ["M99"]()
     b0 = np.nonzero(cfp[:, 3] == 0)[0]
