@@ -51,6 +51,7 @@ static cdm_lm_args op_lm_args(const cdm::Geom& g, float* params, int max_iters, 
   a.c = op_consts(c);
   a.iters_out = iters;
   a.cost_out = cost;
+  a.queue = nullptr;
   return a;
 }
 

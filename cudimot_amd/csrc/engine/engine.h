@@ -53,6 +53,11 @@ constexpr int cdm_ntri = NPARAMS * (NPARAMS + 1) / 2;
 //                              differences (the reference's NUMERICAL step
 //                              rule) with per-perturbation voxel caches;
 //                              Partial_Derivatives_Cached fills the others
+//   CDM_SIGNAL_PARTIALS        Predicted_Signal_Partials_Cached(npar, P, CFP,
+//                              FixP, vc, mc, d) returns the signal of
+//                              Predicted_Signal_Cached and fills the analytic
+//                              entries of Partial_Derivatives_Cached from one
+//                              evaluation of the shared parts
 // Without them the plug-in's Predicted_Signal / Partial_Derivatives are used.
 #ifdef CDM_VOXEL_CACHE
 constexpr int cdm_vc = CDM_VOXEL_CACHE;
@@ -402,6 +407,7 @@ struct cdm_lm_args {
   cdm_consts c;
   int* iters_out;       // successful iterations per voxel (nullable)
   float* cost_out;      // final cost per voxel (nullable)
+  int* queue;           // work-queue counter (queue kernel only; zero at launch)
 };
 
 __device__ __forceinline__ void cdm_minmax_t(double lo, double hi, double p, double& t) {
@@ -611,8 +617,12 @@ struct cdm_jacobian_setup {
 #pragma unroll
     for (int i = 0; i < cdm_np; ++i) d[i] = (TJ)0;
 #ifdef CDM_NUMERICAL_MASK
+#ifdef CDM_SIGNAL_PARTIALS
+    const TJ s = Predicted_Signal_Partials_Cached(cdm_np, p, cm, fx, vc0, mc, d);
+#else
     const TJ s = cdm_signal(p, cm, fx, vc0, mc);
     Partial_Derivatives_Cached(cdm_np, p, cm, fx, vc0, mc, d);
+#endif
 #pragma unroll
     for (int i = 0; i < cdm_np; ++i) {
       if (!((CDM_NUMERICAL_MASK >> i) & 1u)) continue;
@@ -639,13 +649,11 @@ struct cdm_jacobian_setup {
 // tiles it takes from a work queue.  A body stages the CFP table (two block
 // barriers, reached by every thread), then returns early for threads past
 // the last voxel; it has no barrier after that point.
+//
+// One voxel's fit by one voxel group (lane = the thread's lane in it).
 template <int L>
-__device__ __forceinline__ void cdm_levmar_body(const cdm_lm_args& a, int marquardt, int block) {
-  extern __shared__ float s_cfp[];
-  const CDM_F64* s_mc = cdm_stage_cfp(a.g, s_cfp);
-  const int v = block * (cdm_block / L) + threadIdx.x / L;
-  const int lane = threadIdx.x % L;
-  if (v >= a.g.nvox) return;
+__device__ __forceinline__ void cdm_levmar_voxel(const cdm_lm_args& a, int marquardt, int v, int lane, float* s_cfp,
+                                                 const CDM_F64* s_mc) {
   const float* mv = a.g.meas + (size_t)v * a.g.nmeas;
   cdm_tf fx[cdm_fixp_max];
   cdm_load_fixp(a.g, v, fx);
@@ -796,17 +804,65 @@ __device__ __forceinline__ void cdm_levmar_body(const cdm_lm_args& a, int marqua
     if (end) break;
   }
   FixConstraintsLM(cdm_np, p);
-  if (lane == 0) {
+  // Every lane of the group holds the same results and stores them (the
+  // same values to the same addresses): no branch on the lane index, which
+  // in the work-queue loop would let the compiler split the group's lanes
+  // across loop iterations.
+  (void)lane;
 #pragma unroll
-    for (int i = 0; i < cdm_np; ++i) a.params[(size_t)v * cdm_np + i] = (float)p[i];
-    if (a.iters_out) a.iters_out[v] = nsucc;
-    if (a.cost_out) a.cost_out[v] = (float)pcf;
+  for (int i = 0; i < cdm_np; ++i) a.params[(size_t)v * cdm_np + i] = (float)p[i];
+  if (a.iters_out) a.iters_out[v] = nsucc;
+  if (a.cost_out) a.cost_out[v] = (float)pcf;
+}
+
+// Static mapping: voxel group g of block `block` fits voxel block * groups + g.
+template <int L>
+__device__ __forceinline__ void cdm_levmar_body(const cdm_lm_args& a, int marquardt, int block) {
+  extern __shared__ float s_cfp[];
+  const CDM_F64* s_mc = cdm_stage_cfp(a.g, s_cfp);
+  const int v = block * (cdm_block / L) + threadIdx.x / L;
+  const int lane = threadIdx.x % L;
+  if (v >= a.g.nvox) return;
+  cdm_levmar_voxel<L>(a, marquardt, v, lane, s_cfp, s_mc);
+}
+
+// Work queue: LM iteration counts vary widely between voxels (from a few to
+// the --iterLevMar cap), so a group that finishes takes the next voxel from
+// a global counter instead of idling until the other groups of its wave
+// finish theirs.  A voxel's arithmetic does not depend on which group fits
+// it.  The grid holds only the blocks that can be resident at once; every
+// group leaves when the counter passes the last voxel, and after at most
+// nvox voxels in any case.
+//
+// The loop body has no branch on the lane index: every lane of the group
+// executes the fetch (lane 0 adds 1, the others 0) and the broadcast takes
+// lane 0's ticket.  A lane-0-only fetch at the top of the loop together
+// with the lane-0-only stores at the bottom let the compiler form an inner
+// loop in which the other lanes came back to the broadcast without lane 0
+// having fetched, re-reading a stale index.
+template <int L>
+__device__ __forceinline__ void cdm_levmar_queue_body(const cdm_lm_args& a, int marquardt) {
+  extern __shared__ float s_cfp[];
+  const CDM_F64* s_mc = cdm_stage_cfp(a.g, s_cfp);
+  const int lane = threadIdx.x % L;
+  for (int k = 0; k < a.g.nvox; ++k) {
+    const int ticket = atomicAdd(a.queue, lane == 0 ? 1 : 0);
+    const int v = cdm::Group<L>::bcast(ticket, 0);
+    if (v >= a.g.nvox) break;
+    cdm_levmar_voxel<L>(a, marquardt, v, lane, s_cfp, s_mc);
   }
 }
+
+__global__ void cdm_queue_reset_kernel(int* q) { *q = 0; }
 
 // W: resident 256-thread blocks per CU the register allocation must allow
 // (1, 2 or 4; the compiler caps registers and spills beyond them).  Chosen
 // per model and measurement count by the autotuner with L.
+template <int L, int W>
+__global__ void __launch_bounds__(cdm_block, W) cdm_levmar_queue_kernel(cdm_lm_args a, int marquardt) {
+  cdm_levmar_queue_body<L>(a, marquardt);
+}
+
 template <int L, int W>
 __global__ void __launch_bounds__(cdm_block, W) cdm_levmar_kernel(cdm_lm_args a, int marquardt) {
   cdm_levmar_body<L>(a, marquardt, blockIdx.x);

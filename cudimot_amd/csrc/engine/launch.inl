@@ -41,13 +41,48 @@ static hipError_t op_gridsearch(int L, const cdm::Geom& g, float* params, int ng
   return hipGetLastError();
 }
 
+// Resident blocks per CU of the work-queue LM kernel <L, W> at `lds` bytes,
+// queried once per shape (host-side occupancy calculation).
+template <int L, int W>
+static int op_levmar_queue_residency(size_t lds) {
+  static int cached = 0;
+  static size_t cached_lds = 0;
+  if (cached <= 0 || cached_lds != lds) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (cdm_levmar_queue_kernel<L, W>), cdm_block, lds) !=
+            hipSuccess || n <= 0)
+      n = W;
+    cached = n;
+    cached_lds = lds;
+  }
+  return cached;
+}
+
+template <int L, int W>
+static void op_levmar_queue_launch(const cdm_lm_args& a, int marquardt, int nvox, size_t lds, int ncu, hipStream_t s) {
+  const unsigned need = op_grid(nvox, L).x;
+  const unsigned cap = (unsigned)op_levmar_queue_residency<L, W>(lds) * (unsigned)(ncu > 0 ? ncu : 1);
+  hipLaunchKernelGGL((cdm_levmar_queue_kernel<L, W>), dim3(need < cap ? need : cap), dim3(cdm_block), lds, s, a,
+                     marquardt);
+}
+
+// queue: a device int of the model handle; non-null selects the work-queue
+// kernel (grid: the resident blocks, at most one per voxel group), null the
+// static mapping.  ncu: compute units of the device.
 static hipError_t op_levmar(int L, int W, const cdm::Geom& g, float* params, int max_iters, int marquardt,
-                            const cdm::Consts& c, int* iters, float* cost, hipStream_t s) {
+                            const cdm::Consts& c, int* iters, float* cost, int* queue, int ncu, hipStream_t s) {
   if (g.nvox <= 0) return hipSuccess;
   if (g.cfp_ts > cdm_cfp_max || g.fixp_ts > cdm_fixp_max) return hipErrorInvalidValue;
-  const cdm_lm_args a = op_lm_args(g, params, max_iters, c, iters, cost);
-  CDM_DISPATCH_L(L, CDM_DISPATCH_W(W, hipLaunchKernelGGL((cdm_levmar_kernel<LG, WG>), op_grid(g.nvox, LG),
-                                                         dim3(cdm_block), op_lds(g), s, a, marquardt)));
+  cdm_lm_args a = op_lm_args(g, params, max_iters, c, iters, cost);
+  if (!queue) {
+    CDM_DISPATCH_L(L, CDM_DISPATCH_W(W, hipLaunchKernelGGL((cdm_levmar_kernel<LG, WG>), op_grid(g.nvox, LG),
+                                                           dim3(cdm_block), op_lds(g), s, a, marquardt)));
+    return hipGetLastError();
+  }
+  a.queue = queue;
+  hipLaunchKernelGGL(cdm_queue_reset_kernel, dim3(1), dim3(1), 0, s, queue);
+  const size_t lds = op_lds(g);
+  CDM_DISPATCH_L(L, CDM_DISPATCH_W(W, (op_levmar_queue_launch<LG, WG>(a, marquardt, g.nvox, lds, ncu, s))));
   return hipGetLastError();
 }
 

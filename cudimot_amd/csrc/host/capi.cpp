@@ -41,6 +41,8 @@ struct cdm_model {
   // MCMC chain state
   void* d_state = nullptr;
   size_t state_bytes = 0;
+  // LM work-queue counter (one int; calls on one handle are serialised)
+  int* d_queue = nullptr;
 };
 
 namespace {
@@ -370,6 +372,7 @@ void cdm_model_close(cdm_model* m) {
   if (!m) return;
   free_grid(m);
   if (m->d_state) (void)hipFree(m->d_state);
+  if (m->d_queue) (void)hipFree(m->d_queue);
   delete m;
 }
 
@@ -500,9 +503,20 @@ int cdm_levmar(cdm_model* m, int nvox, int nmeas, const float* meas, float* para
   if (max_iters < 0) return fail("--iterLevMar must be >= 0");
   if (nvox == 0) return 0;
   const cdm::Geom g = make_geom(m, nvox, nmeas, meas, cfp, fixp);
+  if (!m->d_queue) {
+    const hipError_t ea = hipMalloc((void**)&m->d_queue, sizeof(int));
+    if (ea != hipSuccess) {
+      m->d_queue = nullptr;
+      return hip_fail(ea, "Levenberg-Marquardt work queue");
+    }
+  }
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                              hipSuccess)
+    ncu = 0;
   const hipError_t e = m->ops->levmar(group_for(m, cdm::kLevMar, nmeas, nvox), occ_for(m, cdm::kLevMar, nmeas), g,
-                                      params, max_iters, marquardt ? 1 : 0,
-                                      make_consts(m), iters, cost, (hipStream_t)stream);
+                                      params, max_iters, marquardt ? 1 : 0, make_consts(m), iters, cost, m->d_queue,
+                                      ncu, (hipStream_t)stream);
   return e == hipSuccess ? 0 : hip_fail(e, "Levenberg-Marquardt launch");
 }
 

@@ -57,8 +57,10 @@ struct ModelOps {
                            const float* grid_f, const double* grid_d, const int* comb_id, const Consts& c,
                            int* kept, hipStream_t s);
   // W: register budget, in resident 256-thread blocks per CU (1, 2 or 4)
+  // queue: device int for the work-queue kernel (null: static voxel mapping);
+  // ncu: compute units of the device
   hipError_t (*levmar)(int L, int W, const Geom& g, float* params, int max_iters, int marquardt, const Consts& c,
-                       int* iters, float* cost, hipStream_t s);
+                       int* iters, float* cost, int* queue, int ncu, hipStream_t s);
   hipError_t (*mcmc)(int L, int W, int rician, const Geom& g, const McmcLaunch& m, const Consts& c, hipStream_t s);
   hipError_t (*predicted)(int L, const Geom& g, int nsamples, const float* samples, float* out, hipStream_t s);
   hipError_t (*bic_aic)(int L, int rician, const Geom& g, int nsamples, const float* samples,
