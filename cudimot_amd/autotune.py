@@ -95,6 +95,8 @@ def tune_model(model, n=65536, reps=2):
                 if best is None or ms < best[1]:
                     best = (L, ms, W)
         res[method] = best
+        print("  %s %s: lanes %d, %.3f ms, budget %d" % (model, METHOD_NAMES[method], best[0], best[1], best[2]),
+              flush=True)
     return cfp.shape[0], res
 
 
