@@ -1222,7 +1222,9 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
         p[i] = old;
       }
     }
-    if (a.recording && (it % a.sampleevery) == 0 && lane == 0) {
+    // every lane of the group holds the same state and stores it (no branch
+    // on the lane index inside the jump loop)
+    if (a.recording && (it % a.sampleevery) == 0) {
       const int k = it / a.sampleevery;
 #pragma unroll
       for (int i = 0; i < cdm_np; ++i) a.samples[((size_t)v * cdm_np + i) * a.nsamples + k] = (float)p[i];

@@ -77,8 +77,10 @@ __device__ inline CDM_F64 bingham_hyp(CDM_F64 x1, CDM_F64 x2, CDM_F64 x3) {
 //     of the cubic's roots evaluated in single precision, with acos from
 //     Abramowitz & Stegun 4.4.45 (absolute error 7e-5) and the hardware
 //     cosine: its error is far below the 1/2 margin.
-__device__ inline CDM_F64 bingham_hyp_rank1(CDM_F64 beta, CDM_F64 kappa, CDM_F64 r, CDM_F64 gx, CDM_F64 gy,
-                                           CDM_F64 gn) {
+// The result is 1F1 = sqrt(c) exp(x) / 4 with c = 2 e3 / K2 and x = T2 - t;
+// bingham_hyp_rank1_parts returns (c, x), bingham_hyp_rank1 the value.
+__device__ inline void bingham_hyp_rank1_parts(CDM_F64 beta, CDM_F64 kappa, CDM_F64 r, CDM_F64 gx, CDM_F64 gy,
+                                               CDM_F64 gn, CDM_F64& c_out, CDM_F64& x_out) {
   const CDM_F64 d2 = -beta, d3 = -kappa;
   const CDM_F64 w1 = gx * gx, w2 = gy * gy, w3 = gn * gn;
   const CDM_F64 s1 = d2 + d3 + r * (w1 + w2 + w3);
@@ -115,13 +117,21 @@ __device__ inline CDM_F64 bingham_hyp_rank1(CDM_F64 beta, CDM_F64 kappa, CDM_F64
   // K2 = p2 / 2, K3 = p3, K4 = 3 p4: T2 = K4 / (8 K2^2) - 5 K3^2 / (24 K2^3)
   const CDM_F64 iK2 = 2.0 * trcp_iter(p2);
   const CDM_F64 T2 = iK2 * iK2 * (0.375 * p4 - (5.0 / 24.0) * p3 * p3 * iK2);
-  return 0.25 * sqrt(2.0 * e3 * iK2) * exp(T2 - t);
+  c_out = 2.0 * e3 * iK2;
+  x_out = T2 - t;
+}
+
+__device__ inline CDM_F64 bingham_hyp_rank1(CDM_F64 beta, CDM_F64 kappa, CDM_F64 r, CDM_F64 gx, CDM_F64 gy,
+                                           CDM_F64 gn) {
+  CDM_F64 c, x;
+  bingham_hyp_rank1_parts(beta, kappa, r, gx, gy, gn, c, x);
+  return 0.25 * sqrt(c) * exp(x);
 }
 
 // Per-fan quantities that depend only on (fintra, kappa, beta): the
 // normaliser 1F1(kappa, beta, 0) and the hindered diffusivities.
 struct bingham_fan {
-  CDM_F64 nc;                      // 1F1(kappa, beta, 0)
+  CDM_F64 lnc4;                    // log(4 1F1(kappa, beta, 0))
   CDM_F64 dpar, dperp1, dperp2;    // hindered diffusivities
 };
 
@@ -140,9 +150,9 @@ __device__ inline void bingham_fan_combine(CDM_F64 fintra, const CDM_F64* h, bin
   const CDM_F64 dperp = CDM_NODDI_DPAR * (1.0 - fintra);
   const CDM_F64 dd = CDM_NODDI_DPAR - dperp;
   const CDM_F64 delta = bingham_fd_delta;
-  f.nc = h[0];
-  f.dpar = dperp + dd * ((h[1] - h[2]) / (2.0 * delta)) / f.nc;
-  f.dperp1 = dperp + dd * ((h[3] - h[4]) / (2.0 * delta)) / f.nc;
+  f.lnc4 = log(4.0 * h[0]);
+  f.dpar = dperp + dd * ((h[1] - h[2]) / (2.0 * delta)) / h[0];
+  f.dperp1 = dperp + dd * ((h[3] - h[4]) / (2.0 * delta)) / h[0];
   f.dperp2 = CDM_NODDI_DPAR + 2 * dperp - f.dpar - f.dperp1;
 }
 
@@ -178,13 +188,18 @@ MACRO T bingham_fan_extra_local(T gx, T gy, T gn, T b, const bingham_fan& f) {
 }
 
 // Intra-axonal compartment of one fan from the local gradient: 1F1 of
-// diag(0, beta, kappa) - b Dpar g_local g_local^T over 1F1(kappa, beta, 0).
+// diag(0, beta, kappa) - b Dpar g_local g_local^T over 1F1(kappa, beta, 0),
+// i.e. sqrt(c) exp(x - log(4 nc)).  The saddle point, c and x are FP64; the
+// square root and the exponential of the O(1) exponent difference are taken
+// in the plug-in's precision T, the precision of the signal itself (FP64 in
+// the LM Jacobian pass).
 MACRO T bingham_fan_intra_local(T kappa, T beta, T gx, T gy, T gn, T b, const bingham_fan& f) {
   const CDM_F64 r = (CDM_F64)b * CDM_NODDI_DPAR;
   // b = 0: the numerator is the normaliser itself, 1F1(diag(0, beta, kappa))
   if (!(r > 0.0)) return (T)1;
-  const CDM_F64 hyp = bingham_hyp_rank1((CDM_F64)beta, (CDM_F64)kappa, r, (CDM_F64)gx, (CDM_F64)gy, (CDM_F64)gn);
-  return (T)(hyp / f.nc);
+  CDM_F64 c, x;
+  bingham_hyp_rank1_parts((CDM_F64)beta, (CDM_F64)kappa, r, (CDM_F64)gx, (CDM_F64)gy, (CDM_F64)gn, c, x);
+  return tsqrt((T)c) * texp((T)(x - f.lnc4));
 }
 
 MACRO T bingham_fan_extra(T th, T ph, T psi, const bingham_fan& f, T* CFP) {

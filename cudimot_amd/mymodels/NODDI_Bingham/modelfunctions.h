@@ -25,7 +25,7 @@
 MACRO void Voxel_Cache(int npar, T* P, T* FixP, CDM_F64* vc) {
   bingham_fan f;
   bingham_fan_setup((CDM_F64)P[1], (CDM_F64)P[2], (CDM_F64)P[3], f);
-  vc[0] = f.nc;
+  vc[0] = f.lnc4;
   vc[1] = f.dpar;
   vc[2] = f.dperp1;
   vc[3] = f.dperp2;
@@ -38,7 +38,7 @@ MACRO CDM_F64 Voxel_Cache_Job(int job, int npar, T* P, T* FixP) {
 MACRO void Voxel_Cache_Combine(int npar, T* P, T* FixP, const CDM_F64* jobs, CDM_F64* vc) {
   bingham_fan f;
   bingham_fan_combine((CDM_F64)P[1], jobs, f);
-  vc[0] = f.nc;
+  vc[0] = f.lnc4;
   vc[1] = f.dpar;
   vc[2] = f.dperp1;
   vc[3] = f.dperp2;
@@ -46,7 +46,7 @@ MACRO void Voxel_Cache_Combine(int npar, T* P, T* FixP, const CDM_F64* jobs, CDM
 
 MACRO void bingham1_parts(T* P, T* CFP, const CDM_F64* vc, T& fan, T& e, T& i) {
   bingham_fan f;
-  f.nc = vc[0];
+  f.lnc4 = vc[0];
   f.dpar = vc[1];
   f.dperp1 = vc[2];
   f.dperp2 = vc[3];
@@ -63,7 +63,7 @@ MACRO T Predicted_Signal_Cached(int npar, T* P, T* CFP, T* FixP, const CDM_F64* 
 
 MACRO void Meas_Voxel_Cache(int npar, T* P, T* CFP, T* FixP, const CDM_F64* vc, const CDM_F64* mc, T* mvc) {
   bingham_fan f;
-  f.nc = vc[0];
+  f.lnc4 = vc[0];
   f.dpar = vc[1];
   f.dperp1 = vc[2];
   f.dperp2 = vc[3];
@@ -72,7 +72,7 @@ MACRO void Meas_Voxel_Cache(int npar, T* P, T* CFP, T* FixP, const CDM_F64* vc, 
 
 MACRO T Predicted_Signal_MV(int npar, T* P, T* CFP, T* FixP, const CDM_F64* vc, const CDM_F64* mc, const T* mvc) {
   bingham_fan f;
-  f.nc = vc[0];
+  f.lnc4 = vc[0];
   f.dpar = vc[1];
   f.dperp1 = vc[2];
   f.dperp2 = vc[3];

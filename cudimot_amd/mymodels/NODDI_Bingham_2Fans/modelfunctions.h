@@ -15,11 +15,11 @@ MACRO void Voxel_Cache(int npar, T* P, T* FixP, CDM_F64* vc) {
   bingham_fan f1, f2;
   bingham_fan_setup((CDM_F64)P[2], (CDM_F64)P[3], (CDM_F64)P[4], f1);
   bingham_fan_setup((CDM_F64)P[8], (CDM_F64)P[9], (CDM_F64)P[10], f2);
-  vc[0] = f1.nc;
+  vc[0] = f1.lnc4;
   vc[1] = f1.dpar;
   vc[2] = f1.dperp1;
   vc[3] = f1.dperp2;
-  vc[4] = f2.nc;
+  vc[4] = f2.lnc4;
   vc[5] = f2.dpar;
   vc[6] = f2.dperp1;
   vc[7] = f2.dperp2;
@@ -27,11 +27,11 @@ MACRO void Voxel_Cache(int npar, T* P, T* FixP, CDM_F64* vc) {
 
 MACRO T Predicted_Signal_Cached(int npar, T* P, T* CFP, T* FixP, const CDM_F64* vc, const CDM_F64* mc) {
   bingham_fan f1, f2;
-  f1.nc = vc[0];
+  f1.lnc4 = vc[0];
   f1.dpar = vc[1];
   f1.dperp1 = vc[2];
   f1.dperp2 = vc[3];
-  f2.nc = vc[4];
+  f2.lnc4 = vc[4];
   f2.dpar = vc[5];
   f2.dperp1 = vc[6];
   f2.dperp2 = vc[7];
