@@ -253,6 +253,7 @@ def build(models=None, extra_dirs=(), force=False, jobs=None, batches=None):
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-ldl"])
     os.replace(tmp, LIB)
     build_cli()
+    build_opcount(models)
     return LIB
 
 
@@ -337,7 +338,7 @@ def main(argv=None):
     models = [m for m in a.models.split(",") if m] or None
     batches = dict((b.split("=", 1)[0], b.split("=", 1)[1].split(",")) for b in a.batch) if a.batch else None
     print(build(models, a.extra_model_dir, a.force, batches=batches))
-    print(build_opcount(models))
+    print(OPC_LIB)
 
 
 if __name__ == "__main__":

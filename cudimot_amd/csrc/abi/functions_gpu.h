@@ -98,8 +98,10 @@ CDM_FUNC double terf(double x) { return erf(x); }
 // a few radians), cosf on the host.
 #if defined(__HIP_DEVICE_COMPILE__)
 CDM_FUNC float tcos_start(float x) { return __cosf(x); }
+CDM_FUNC float trsq_start(float x) { return __builtin_amdgcn_rsqf(x); }
 #else
 CDM_FUNC float tcos_start(float x) { return cosf(x); }
+CDM_FUNC float trsq_start(float x) { return 1.0f / sqrtf(x); }
 #endif
 
 #if defined(__HIP_DEVICE_COMPILE__)

@@ -76,7 +76,8 @@ __device__ inline CDM_F64 bingham_hyp(CDM_F64 x1, CDM_F64 x2, CDM_F64 x3) {
 //   * l_min only places the start, so it comes from the trigonometric form
 //     of the cubic's roots evaluated in single precision, with acos from
 //     Abramowitz & Stegun 4.4.45 (absolute error 7e-5) and the hardware
-//     cosine: its error is far below the 1/2 margin.
+//     cosine and reciprocal square root: its error is far below the 1/2
+//     margin.
 // The result is 1F1 = sqrt(c) exp(x) / 4 with c = 2 e3 / K2 and x = T2 - t;
 // bingham_hyp_rank1_parts returns (c, x), bingham_hyp_rank1 the value.
 __device__ inline void bingham_hyp_rank1_parts(CDM_F64 beta, CDM_F64 kappa, CDM_F64 r, CDM_F64 gx, CDM_F64 gy,
@@ -90,8 +91,10 @@ __device__ inline void bingham_hyp_rank1_parts(CDM_F64 beta, CDM_F64 kappa, CDM_
   CDM_F64 pd = (s1 * s1 - 3.0 * s2) * (1.0 / 9.0);
   if (pd < 0.0) pd = 0.0;
   const CDM_F64 hd = (s1 * (2.0 * s1 * s1 - 9.0 * s2) + 27.0 * s3) * (1.0 / 54.0);
-  const float p = (float)pd, sp = sqrtf(p);
-  float c = sp > 0.0f ? (float)hd / (p * sp) : 0.0f;
+  const float p = (float)pd;
+  const float rp = p > 1e-30f ? trsq_start(p) : 0.0f;  // 1 / sqrt(p), hardware estimate
+  const float sp = p * rp;
+  float c = (float)hd * (rp * rp * rp);
   c = fminf(fmaxf(c, -1.0f), 1.0f);
   const float ca = fabsf(c);
   float ac = sqrtf(1.0f - ca) * (1.5707288f + ca * (-0.2121144f + ca * (0.0742610f + ca * -0.0187293f)));

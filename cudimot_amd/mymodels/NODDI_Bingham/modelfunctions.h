@@ -21,6 +21,7 @@
 #define CDM_MEAS_VOXEL_CACHE 1              // intra-axonal signal per measurement
 #define CDM_MEAS_VOXEL_CACHE_DEPENDS 0x7Cu  // kappa, beta, th, ph, psi
 #define CDM_SIGNAL_PARTIALS 1               // signal and analytic entries together
+#define CDM_MEAS_CACHE 1                    // exp(-b Diso) per measurement
 
 MACRO void Voxel_Cache(int npar, T* P, T* FixP, CDM_F64* vc) {
   bingham_fan f;
@@ -30,6 +31,9 @@ MACRO void Voxel_Cache(int npar, T* P, T* FixP, CDM_F64* vc) {
   vc[2] = f.dperp1;
   vc[3] = f.dperp2;
 }
+
+// The isotropic compartment depends on the b-value only.
+MACRO void Meas_Cache(T* CFP, CDM_F64* mc) { mc[0] = exp(-(CDM_F64)CFP[3] * CDM_NODDI_DISO); }
 
 MACRO CDM_F64 Voxel_Cache_Job(int job, int npar, T* P, T* FixP) {
   return bingham_fan_job(job, (CDM_F64)P[2], (CDM_F64)P[3]);
@@ -57,7 +61,7 @@ MACRO void bingham1_parts(T* P, T* CFP, const CDM_F64* vc, T& fan, T& e, T& i) {
 MACRO T Predicted_Signal_Cached(int npar, T* P, T* CFP, T* FixP, const CDM_F64* vc, const CDM_F64* mc) {
   T fan, e, i;
   bingham1_parts(P, CFP, vc, fan, e, i);
-  const T iso = exp_gpu(-CFP[3] * (T)CDM_NODDI_DISO);
+  const T iso = (T)mc[0];
   return FixP[0] * (((T)1 - P[0]) * fan + P[0] * iso);
 }
 
@@ -78,14 +82,15 @@ MACRO T Predicted_Signal_MV(int npar, T* P, T* CFP, T* FixP, const CDM_F64* vc, 
   f.dperp2 = vc[3];
   const T e = bingham_fan_extra(P[4], P[5], P[6], f, CFP);
   const T fan = ((T)1 - P[1]) * e + P[1] * mvc[0];
-  const T iso = exp_gpu(-CFP[3] * (T)CDM_NODDI_DISO);
+  const T iso = (T)mc[0];
   return FixP[0] * (((T)1 - P[0]) * fan + P[0] * iso);
 }
 
 MACRO T Predicted_Signal(int npar, T* P, T* CFP, T* FixP) {
-  CDM_F64 vc[4];
+  CDM_F64 vc[4], mc[1];
   Voxel_Cache(npar, P, FixP, vc);
-  return Predicted_Signal_Cached(npar, P, CFP, FixP, vc, (const CDM_F64*)0);
+  Meas_Cache(CFP, mc);
+  return Predicted_Signal_Cached(npar, P, CFP, FixP, vc, mc);
 }
 
 // kappa must exceed beta by 0.05 (NODDI_Bingham/modelfunctions.h:169-178).
@@ -96,7 +101,7 @@ MACRO void Partial_Derivatives_Cached(int npar, T* P, T* CFP, T* FixP, const CDM
                                       T* derivatives) {
   T fan, e, i;
   bingham1_parts(P, CFP, vc, fan, e, i);
-  const T iso = exp_gpu(-CFP[3] * (T)CDM_NODDI_DISO);
+  const T iso = (T)mc[0];
   derivatives[0] = FixP[0] * (iso - fan);
   derivatives[1] = FixP[0] * ((T)1 - P[0]) * (i - e);
 }
@@ -107,16 +112,17 @@ MACRO T Predicted_Signal_Partials_Cached(int npar, T* P, T* CFP, T* FixP, const 
                                          T* derivatives) {
   T fan, e, i;
   bingham1_parts(P, CFP, vc, fan, e, i);
-  const T iso = exp_gpu(-CFP[3] * (T)CDM_NODDI_DISO);
+  const T iso = (T)mc[0];
   derivatives[0] = FixP[0] * (iso - fan);
   derivatives[1] = FixP[0] * ((T)1 - P[0]) * (i - e);
   return FixP[0] * (((T)1 - P[0]) * fan + P[0] * iso);
 }
 
 MACRO void Partial_Derivatives(int npar, T* P, T* CFP, T* FixP, T* derivatives) {
-  CDM_F64 vc[4];
+  CDM_F64 vc[4], mc[1];
   Voxel_Cache(npar, P, FixP, vc);
-  Partial_Derivatives_Cached(npar, P, CFP, FixP, vc, (const CDM_F64*)0, derivatives);
+  Meas_Cache(CFP, mc);
+  Partial_Derivatives_Cached(npar, P, CFP, FixP, vc, mc, derivatives);
   derivatives[2] = NUMERICAL(2);
   derivatives[3] = NUMERICAL(3);
   derivatives[4] = NUMERICAL(4);
