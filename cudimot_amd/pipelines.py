@@ -13,6 +13,12 @@ GridSearch step:
        (--fixed=4,5, grid of Pipeline_NODDI_Bingham.sh:204-208), then LM with
        the same parameters fixed (the script's step 1 runs LM by default);
     3. LM of all parameters but fiso (--fixed=0).
+  GridSearch + LM (Watson), Pipeline_NODDI_Watson.sh:185-246:
+    1. tensor fit -> theta, phi of V1;
+    2. GridSearch over fiso, fintra, kappa (grid :200-202) with theta, phi
+       fixed (--fixed=3,4, --no_LevMar);
+    3. LM of the fractions alone (--fixed=2,3,4);
+    4. LM of all parameters.
   Learned + LM:
     1. the network predicts fiso, fintra, kappa, beta, theta, phi;
     2. psi from a 12-point scan (GridSearch over psi alone);
@@ -98,6 +104,25 @@ def bingham_gridsearch_lm(meas, cfp, s0, lm_iters=200, grid=BINGHAM_GRID, frame=
     step2 = _model("NODDI_Bingham", fixed="0")
     cost = torch.empty(n, device=meas.device)
     step2.levmar(meas, p, d_cfp, s0, max_iters=lm_iters, cost=cost, stream=stream)
+    return p, cost
+
+
+def watson_gridsearch_lm(meas, cfp, s0, lm_iters=200, grid=WATSON_GRID, frame=None, kept=None, stream=None):
+    """GridSearch + LM recipe of the reference's Watson pipeline; returns
+    (params [n][5], cost [n]) on the device.  kept (device int [n],
+    optional) receives the GridSearch combination per voxel."""
+    torch = _torch()
+    n = meas.shape[0]
+    d_cfp = _dev(cfp)
+    if frame is None:
+        frame = tensor_frame(meas, cfp, stream=stream)
+    spec = capi.Model("NODDI_Watson").spec()
+    p = _dev(np.tile(np.asarray(spec["p_init"], dtype=np.float32), (n, 1)))
+    p[:, 3:5] = frame[:, 0:2]
+    _model("NODDI_Watson", fixed="3,4", grid=grid).gridsearch(meas, p, d_cfp, s0, kept=kept, stream=stream)
+    _model("NODDI_Watson", fixed="2,3,4").levmar(meas, p, d_cfp, s0, max_iters=lm_iters, stream=stream)
+    cost = torch.empty(n, device=meas.device)
+    capi.Model("NODDI_Watson").levmar(meas, p, d_cfp, s0, max_iters=lm_iters, cost=cost, stream=stream)
     return p, cost
 
 
