@@ -66,7 +66,8 @@ __device__ inline CDM_F64 bingham_hyp(CDM_F64 x1, CDM_F64 x2, CDM_F64 x3) {
 //     E2 = s2 - 2 s1 s + 3 s^2).  Newton's identities give the power sums
 //     p_j = sum_i y_i^j (p1 = e1, p2 = e1 p1 - 2 e2, p3 = e1 p2 - e2 p1 + 3 e3,
 //     p4 = e1 p3 - e2 p2 + e3 p1), and K' = p1 / 2, K'' = p2 / 2, K''' = p3,
-//     K4 = 3 p4;
+//     K4 = 3 p4; scaled by powers of E they are polynomials in E, E1, E2,
+//     which leaves one reciprocal per Halley step and one for the result;
 //   * the saddle point t < l_min solves K'(t) = 1, so it lies in
 //     [l_min - 3/2, l_min - 1/2].  Three Halley steps on K'(t) - 1 from
 //     l_min - 1/2 reach it to double precision (worst relative error of the
@@ -100,27 +101,32 @@ __device__ inline void bingham_hyp_rank1_parts(CDM_F64 beta, CDM_F64 kappa, CDM_
   float ac = sqrtf(1.0f - ca) * (1.5707288f + ca * (-0.2121144f + ca * (0.0742610f + ca * -0.0187293f)));
   if (c < 0.0f) ac = 3.14159265f - ac;
   const float lmin = (float)(s1 * (1.0 / 3.0)) + 2.0f * sp * tcos_start((ac + 6.28318531f) * (1.0f / 3.0f));
+  // In terms of E, A = E2 and B = E1 (so e1 = A/E, e2 = B/E, e3 = 1/E):
+  // D = E^2 p2 = A^2 - 2 B E, P3 = E^3 p3 = A (A^2 - 3 B E) + 3 E^2 and
+  // E^4 p4 = A P3 - B E D + A E^2.  With f = K' - 1 = (A - 2E) / (2E) the
+  // Halley step 2 f K'' / (2 K''^2 - f K''') is (A - 2E) D E / (D^2 - (A - 2E) P3):
+  // one reciprocal per step.
   CDM_F64 t = (CDM_F64)lmin - 0.5;
   for (int it = 0; it < 3; ++it) {
-    const CDM_F64 iE = trcp_iter(s3 - t * (s2 - t * (s1 - t)));
-    const CDM_F64 e1 = (s2 - t * (2.0 * s1 - 3.0 * t)) * iE;
-    const CDM_F64 e2 = (s1 - 3.0 * t) * iE;
-    const CDM_F64 p2 = e1 * e1 - 2.0 * e2;
-    const CDM_F64 p3 = e1 * p2 - e2 * e1 + 3.0 * iE;
-    const CDM_F64 f = 0.5 * e1 - 1.0, df = 0.5 * p2;  // K' - 1 and K''; K''' = p3
-    t -= 2.0 * f * df * trcp_iter(2.0 * df * df - f * p3);
+    const CDM_F64 E = s3 - t * (s2 - t * (s1 - t));
+    const CDM_F64 A = s2 - t * (2.0 * s1 - 3.0 * t);
+    const CDM_F64 B = s1 - 3.0 * t;
+    const CDM_F64 D = A * A - 2.0 * B * E;
+    const CDM_F64 P3 = A * (A * A - 3.0 * B * E) + 3.0 * E * E;
+    const CDM_F64 f2 = A - 2.0 * E;
+    t -= f2 * D * E * trcp_iter(D * D - f2 * P3);
   }
   const CDM_F64 E = s3 - t * (s2 - t * (s1 - t));
-  const CDM_F64 e3 = trcp_iter(E);
-  const CDM_F64 e1 = (s2 - t * (2.0 * s1 - 3.0 * t)) * e3;
-  const CDM_F64 e2 = (s1 - 3.0 * t) * e3;
-  const CDM_F64 p2 = e1 * e1 - 2.0 * e2;
-  const CDM_F64 p3 = e1 * p2 - e2 * e1 + 3.0 * e3;
-  const CDM_F64 p4 = e1 * p3 - e2 * p2 + e3 * e1;
-  // K2 = p2 / 2, K3 = p3, K4 = 3 p4: T2 = K4 / (8 K2^2) - 5 K3^2 / (24 K2^3)
-  const CDM_F64 iK2 = 2.0 * trcp_iter(p2);
-  const CDM_F64 T2 = iK2 * iK2 * (0.375 * p4 - (5.0 / 24.0) * p3 * p3 * iK2);
-  c_out = 2.0 * e3 * iK2;
+  const CDM_F64 A = s2 - t * (2.0 * s1 - 3.0 * t);
+  const CDM_F64 B = s1 - 3.0 * t;
+  const CDM_F64 D = A * A - 2.0 * B * E;
+  const CDM_F64 P3 = A * (A * A - 3.0 * B * E) + 3.0 * E * E;
+  const CDM_F64 P4 = A * P3 - B * E * D + A * E * E;
+  // K2 = D / (2 E^2), K3 = P3 / E^3, K4 = 3 P4 / E^4:
+  // T2 = K4 / (8 K2^2) - 5 K3^2 / (24 K2^3) = (1.5 P4 - (5/3) P3^2 / D) / D^2, 2 e3 / K2 = 4 E / D
+  const CDM_F64 iD = trcp_iter(D);
+  const CDM_F64 T2 = iD * iD * (1.5 * P4 - (5.0 / 3.0) * P3 * P3 * iD);
+  c_out = 4.0 * E * iD;
   x_out = T2 - t;
 }
 
