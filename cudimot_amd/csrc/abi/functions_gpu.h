@@ -99,9 +99,35 @@ CDM_FUNC double terf(double x) { return erf(x); }
 #if defined(__HIP_DEVICE_COMPILE__)
 CDM_FUNC float tcos_start(float x) { return __cosf(x); }
 CDM_FUNC float trsq_start(float x) { return __builtin_amdgcn_rsqf(x); }
+CDM_FUNC float tsqrt_start(float x) { return __builtin_amdgcn_sqrtf(x); }
 #else
 CDM_FUNC float tcos_start(float x) { return cosf(x); }
 CDM_FUNC float trsq_start(float x) { return 1.0f / sqrtf(x); }
+CDM_FUNC float tsqrt_start(float x) { return sqrtf(x); }
+#endif
+
+// A double-precision intermediate finished in the signal's precision T
+// (engine extension):
+//  * texp_of<T>(y) = exp(y).  For float the reduction y log2(e) = n + f,
+//    |f| <= 1/2, is exact enough in double that only the hardware 2^f
+//    (about 1 ulp on that interval) and an exact scaling by 2^n remain in
+//    single precision: no float rounding of a large argument.
+//  * tsqrt_of<T>(c) = sqrt(c) for c a normal positive number: the hardware
+//    square root (1 ulp) for float, sqrt for double.
+template <typename T> CDM_FUNC T texp_of(double y);
+template <typename T> CDM_FUNC T tsqrt_of(double c);
+template <> CDM_FUNC double texp_of<double>(double y) { return exp(y); }
+template <> CDM_FUNC double tsqrt_of<double>(double c) { return sqrt(c); }
+#if defined(__HIP_DEVICE_COMPILE__)
+template <> CDM_FUNC float texp_of<float>(double y) {
+  const double z = y * 1.4426950408889634;  // log2(e)
+  const double n = rint(z);
+  return __builtin_ldexpf(__builtin_amdgcn_exp2f((float)(z - n)), (int)n);
+}
+template <> CDM_FUNC float tsqrt_of<float>(double c) { return __builtin_amdgcn_sqrtf((float)c); }
+#else
+template <> CDM_FUNC float texp_of<float>(double y) { return (float)exp(y); }
+template <> CDM_FUNC float tsqrt_of<float>(double c) { return sqrtf((float)c); }
 #endif
 
 #if defined(__HIP_DEVICE_COMPILE__)

@@ -132,6 +132,8 @@ CDM_ABI_1(texp, exp)
 CDM_ABI_1(terf, erf)
 #undef CDM_ABI_1
 inline cdm_cd trcp_iter(cdm_cd x) { return 1.0 / x; }
+template <typename T> inline T texp_of(cdm_cd y) { return cdm_count::exp(T(y)); }
+template <typename T> inline T tsqrt_of(cdm_cd c) { return cdm_count::sqrt(T(c)); }
 inline cdm_cd tan_gpu(cdm_cf x) { return cdm_count::tan(cdm_cd(x)); }
 inline cdm_cd tan_gpu(cdm_cd x) { return cdm_count::tan(x); }
 inline cdm_cd sqrt_gpu(cdm_cf x) { return cdm_count::sqrt(cdm_cd(x)); }

@@ -77,8 +77,8 @@ __device__ inline CDM_F64 bingham_hyp(CDM_F64 x1, CDM_F64 x2, CDM_F64 x3) {
 //   * l_min only places the start, so it comes from the trigonometric form
 //     of the cubic's roots evaluated in single precision, with acos from
 //     Abramowitz & Stegun 4.4.45 (absolute error 7e-5) and the hardware
-//     cosine and reciprocal square root: its error is far below the 1/2
-//     margin.
+//     cosine, square root and reciprocal square root: its error is far
+//     below the 1/2 margin.
 // The result is 1F1 = sqrt(c) exp(x) / 4 with c = 2 e3 / K2 and x = T2 - t;
 // bingham_hyp_rank1_parts returns (c, x), bingham_hyp_rank1 the value.
 __device__ inline void bingham_hyp_rank1_parts(CDM_F64 beta, CDM_F64 kappa, CDM_F64 r, CDM_F64 gx, CDM_F64 gy,
@@ -98,7 +98,7 @@ __device__ inline void bingham_hyp_rank1_parts(CDM_F64 beta, CDM_F64 kappa, CDM_
   float c = (float)hd * (rp * rp * rp);
   c = fminf(fmaxf(c, -1.0f), 1.0f);
   const float ca = fabsf(c);
-  float ac = sqrtf(1.0f - ca) * (1.5707288f + ca * (-0.2121144f + ca * (0.0742610f + ca * -0.0187293f)));
+  float ac = tsqrt_start(1.0f - ca) * (1.5707288f + ca * (-0.2121144f + ca * (0.0742610f + ca * -0.0187293f)));
   if (c < 0.0f) ac = 3.14159265f - ac;
   const float lmin = (float)(s1 * (1.0 / 3.0)) + 2.0f * sp * tcos_start((ac + 6.28318531f) * (1.0f / 3.0f));
   // In terms of E, A = E2 and B = E1 (so e1 = A/E, e2 = B/E, e3 = 1/E):
@@ -199,16 +199,16 @@ MACRO T bingham_fan_extra_local(T gx, T gy, T gn, T b, const bingham_fan& f) {
 // Intra-axonal compartment of one fan from the local gradient: 1F1 of
 // diag(0, beta, kappa) - b Dpar g_local g_local^T over 1F1(kappa, beta, 0),
 // i.e. sqrt(c) exp(x - log(4 nc)).  The saddle point, c and x are FP64; the
-// square root and the exponential of the O(1) exponent difference are taken
-// in the plug-in's precision T, the precision of the signal itself (FP64 in
-// the LM Jacobian pass).
+// square root and the exponential of the exponent difference are finished in
+// the plug-in's precision T, the precision of the signal itself (FP64 in the
+// LM Jacobian pass), the exponential from its double-precision argument.
 MACRO T bingham_fan_intra_local(T kappa, T beta, T gx, T gy, T gn, T b, const bingham_fan& f) {
   const CDM_F64 r = (CDM_F64)b * CDM_NODDI_DPAR;
   // b = 0: the numerator is the normaliser itself, 1F1(diag(0, beta, kappa))
   if (!(r > 0.0)) return (T)1;
   CDM_F64 c, x;
   bingham_hyp_rank1_parts((CDM_F64)beta, (CDM_F64)kappa, r, (CDM_F64)gx, (CDM_F64)gy, (CDM_F64)gn, c, x);
-  return tsqrt((T)c) * texp((T)(x - f.lnc4));
+  return tsqrt_of<T>(c) * texp_of<T>(x - f.lnc4);
 }
 
 MACRO T bingham_fan_extra(T th, T ph, T psi, const bingham_fan& f, T* CFP) {
