@@ -106,6 +106,22 @@ CDM_FUNC float trsq_start(float x) { return 1.0f / sqrtf(x); }
 CDM_FUNC float tsqrt_start(float x) { return sqrtf(x); }
 #endif
 
+// Fast forms for single-precision terms whose argument already carries
+// float rounding (engine extension): texp_fast(x) is the hardware 2^y of
+// y = x log2(e), with relative error about |x| 2^-24 + 1 ulp, the size of
+// the error the rounded argument itself brings; tdiv_fast(a, b) is a times
+// the hardware reciprocal of b (about 2 ulp, b a normal number).  For
+// double both are the exact functions.
+#if defined(__HIP_DEVICE_COMPILE__)
+CDM_FUNC float texp_fast(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
+CDM_FUNC float tdiv_fast(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+#else
+CDM_FUNC float texp_fast(float x) { return expf(x); }
+CDM_FUNC float tdiv_fast(float a, float b) { return a / b; }
+#endif
+CDM_FUNC double texp_fast(double x) { return exp(x); }
+CDM_FUNC double tdiv_fast(double a, double b) { return a / b; }
+
 // A double-precision intermediate finished in the signal's precision T
 // (engine extension):
 //  * texp_of<T>(y) = exp(y).  For float the reduction y log2(e) = n + f,

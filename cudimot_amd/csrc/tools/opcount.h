@@ -133,6 +133,10 @@ CDM_ABI_1(terf, erf)
 #undef CDM_ABI_1
 inline cdm_cd trcp_iter(cdm_cd x) { return 1.0 / x; }
 template <typename T> inline T texp_of(cdm_cd y) { return cdm_count::exp(T(y)); }
+inline cdm_cf texp_fast(cdm_cf x) { return cdm_count::exp(x); }
+inline cdm_cd texp_fast(cdm_cd x) { return cdm_count::exp(x); }
+inline cdm_cf tdiv_fast(cdm_cf a, cdm_cf b) { return a / b; }
+inline cdm_cd tdiv_fast(cdm_cd a, cdm_cd b) { return a / b; }
 template <typename T> inline T tsqrt_of(cdm_cd c) { return cdm_count::sqrt(T(c)); }
 inline cdm_cd tan_gpu(cdm_cf x) { return cdm_count::tan(cdm_cd(x)); }
 inline cdm_cd tan_gpu(cdm_cd x) { return cdm_count::tan(x); }

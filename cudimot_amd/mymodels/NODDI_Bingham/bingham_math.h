@@ -187,13 +187,15 @@ MACRO void bingham_fan_local(T th, T ph, T psi, T* CFP, T& gx, T& gy, T& gn) {
   gx = CFP[0] * gx_local0 + CFP[1] * gx_local1 + CFP[2] * gx_local2;
 }
 
-// Hindered compartment of one fan from the local gradient.
+// Hindered compartment of one fan from the local gradient (the azimuth
+// ratio and the exponential in their fast forms: both arguments are
+// products of single-precision values already).
 MACRO T bingham_fan_extra_local(T gx, T gy, T gn, T b, const bingham_fan& f) {
   const T c2 = gn * gn, s2 = (T)1 - c2;
   // cos^2 of the azimuth atan2(gy, gx) in the plane normal to the fibre
   const T gxy = gx * gx + gy * gy;
-  const T caz2 = gxy > (T)0 ? gx * gx / gxy : (T)1, saz2 = (T)1 - caz2;
-  return exp_gpu(-b * ((T)f.dpar * c2 + s2 * ((T)f.dperp1 * saz2 + (T)f.dperp2 * caz2)));
+  const T caz2 = gxy > (T)0 ? tdiv_fast(gx * gx, gxy) : (T)1, saz2 = (T)1 - caz2;
+  return texp_fast(-b * ((T)f.dpar * c2 + s2 * ((T)f.dperp1 * saz2 + (T)f.dperp2 * caz2)));
 }
 
 // Intra-axonal compartment of one fan from the local gradient: 1F1 of
