@@ -56,6 +56,18 @@ inline double texp(double x) { return exp(x); }
 inline void tsincos(double x, double* s, double* c) { *s = sin(x); *c = cos(x); }
 inline double terf(double x) { return erf(x); }
 
+// Starting-value and finishing helpers of this project's plug-ins: the
+// starting values in single precision, as on the GPU (they only place the
+// first iterate of a root finder); everything else exact in double.
+inline float tcos_start(float x) { return cosf(x); }
+inline float trsq_start(float x) { return 1.0f / sqrtf(x); }
+inline float tsqrt_start(float x) { return sqrtf(x); }
+inline double trcp_iter(double x) { return 1.0 / x; }
+inline double texp_fast(double x) { return exp(x); }
+inline double tdiv_fast(double a, double b) { return a / b; }
+template <typename T> inline T texp_of(double y) { return (T)exp(y); }
+template <typename T> inline T tsqrt_of(double c) { return (T)sqrt(c); }
+
 #ifndef MACRO
 #define MACRO template <typename T> __device__ inline
 #endif
