@@ -68,17 +68,21 @@ __device__ inline CDM_F64 bingham_hyp(CDM_F64 x1, CDM_F64 x2, CDM_F64 x3) {
 //     p4 = e1 p3 - e2 p2 + e3 p1), and K' = p1 / 2, K'' = p2 / 2, K''' = p3,
 //     K4 = 3 p4; scaled by powers of E they are polynomials in E, E1, E2,
 //     which leaves one reciprocal per Halley step and one for the result;
-//   * the saddle point t < l_min solves K'(t) = 1, so it lies in
-//     [l_min - 3/2, l_min - 1/2].  Three Halley steps on K'(t) - 1 from
-//     l_min - 1/2 reach it to double precision (worst relative error of the
-//     result 6e-13 against an extended-precision bisection over 10^6 cases
-//     across kappa, beta, b and gradient directions, double roots included);
-//     a fixed step count keeps every lane of a wave on the same path;
-//   * l_min only places the start, so it comes from the trigonometric form
-//     of the cubic's roots evaluated in single precision, with acos from
-//     Abramowitz & Stegun 4.4.45 (absolute error 7e-5) and the hardware
-//     cosine, square root and reciprocal square root: its error is far
-//     below the 1/2 margin.
+//   * the saddle point t < l_min solves K'(t) = e1 / 2 = 1, i.e. it is a
+//     root of the cubic G(t) = E2 - 2E = 2t^3 + (3 - 2 s1) t^2
+//     + 2 (s2 - s1) t + (s2 - 2 s3).  On t < l_min, G = 2E (K' - 1) changes
+//     sign exactly once; every other root of G lies at or above l_min, and
+//     the saddle point lies below l_min - 1/2, so it is the smallest root of
+//     G and a simple one;
+//   * that root comes from the trigonometric form of a cubic's roots,
+//     evaluated in single precision (acos from Abramowitz & Stegun 4.4.45,
+//     absolute error 7e-5, and the hardware cosine, square root and
+//     reciprocal square root), and one Halley step on K'(t) - 1 in double
+//     brings it to double precision: worst relative error of the result
+//     7e-13 against an extended-precision bisection over 10^6 cases across
+//     kappa, beta, b and gradient directions, double roots included, the
+//     same as three steps from l_min - 1/2.  The step count is the same on
+//     every lane of a wave.
 // The result is 1F1 = sqrt(c) exp(x) / 4 with c = 2 e3 / K2 and x = T2 - t;
 // bingham_hyp_rank1_parts returns (c, x), bingham_hyp_rank1 the value.
 __device__ inline void bingham_hyp_rank1_parts(CDM_F64 beta, CDM_F64 kappa, CDM_F64 r, CDM_F64 gx, CDM_F64 gy,
@@ -88,10 +92,13 @@ __device__ inline void bingham_hyp_rank1_parts(CDM_F64 beta, CDM_F64 kappa, CDM_
   const CDM_F64 s1 = d2 + d3 + r * (w1 + w2 + w3);
   const CDM_F64 s2 = d2 * d3 + r * (w1 * (d2 + d3) + w2 * d3 + w3 * d2);
   const CDM_F64 s3 = r * w1 * d2 * d3;
-  // smallest root of s^3 - s1 s^2 + s2 s - s3: s1/3 + 2 sqrt(p) cos((acos(c) + 2 pi) / 3)
-  CDM_F64 pd = (s1 * s1 - 3.0 * s2) * (1.0 / 9.0);
+  // G(t) / 2 = t^3 + a t^2 + bq t + c0; its smallest root is
+  // -a/3 + 2 sqrt(p) cos((acos(h / p^3/2) + 2 pi) / 3) with p = (a^2 - 3 bq) / 9
+  // and h = (a (9 bq - 2 a^2) - 27 c0) / 54
+  const CDM_F64 a = 1.5 - s1, bq = s2 - s1, c0 = 0.5 * s2 - s3;
+  CDM_F64 pd = (a * a - 3.0 * bq) * (1.0 / 9.0);
   if (pd < 0.0) pd = 0.0;
-  const CDM_F64 hd = (s1 * (2.0 * s1 * s1 - 9.0 * s2) + 27.0 * s3) * (1.0 / 54.0);
+  const CDM_F64 hd = (a * (9.0 * bq - 2.0 * a * a) - 27.0 * c0) * (1.0 / 54.0);
   const float p = (float)pd;
   const float rp = p > 1e-30f ? trsq_start(p) : 0.0f;  // 1 / sqrt(p), hardware estimate
   const float sp = p * rp;
@@ -100,14 +107,14 @@ __device__ inline void bingham_hyp_rank1_parts(CDM_F64 beta, CDM_F64 kappa, CDM_
   const float ca = fabsf(c);
   float ac = tsqrt_start(1.0f - ca) * (1.5707288f + ca * (-0.2121144f + ca * (0.0742610f + ca * -0.0187293f)));
   if (c < 0.0f) ac = 3.14159265f - ac;
-  const float lmin = (float)(s1 * (1.0 / 3.0)) + 2.0f * sp * tcos_start((ac + 6.28318531f) * (1.0f / 3.0f));
+  const float t0 = (float)(a * (-1.0 / 3.0)) + 2.0f * sp * tcos_start((ac + 6.28318531f) * (1.0f / 3.0f));
   // In terms of E, A = E2 and B = E1 (so e1 = A/E, e2 = B/E, e3 = 1/E):
   // D = E^2 p2 = A^2 - 2 B E, P3 = E^3 p3 = A (A^2 - 3 B E) + 3 E^2 and
   // E^4 p4 = A P3 - B E D + A E^2.  With f = K' - 1 = (A - 2E) / (2E) the
   // Halley step 2 f K'' / (2 K''^2 - f K''') is (A - 2E) D E / (D^2 - (A - 2E) P3):
-  // one reciprocal per step.
-  CDM_F64 t = (CDM_F64)lmin - 0.5;
-  for (int it = 0; it < 3; ++it) {
+  // one reciprocal.
+  CDM_F64 t = (CDM_F64)t0;
+  {
     const CDM_F64 E = s3 - t * (s2 - t * (s1 - t));
     const CDM_F64 A = s2 - t * (2.0 * s1 - 3.0 * t);
     const CDM_F64 B = s1 - 3.0 * t;

@@ -22,10 +22,28 @@ _F = ctypes.POINTER(ctypes.c_float)
 _I = ctypes.POINTER(ctypes.c_int)
 
 
+def _check_fresh(path):
+    """liboracle.so must come from the sources in this tree: its build stamp
+    holds their digest."""
+    import sys
+    sys.path.insert(0, HERE)
+    try:
+        import build_oracle
+    finally:
+        sys.path.pop(0)
+    stamp = path + ".sha"
+    have = open(stamp).read().strip() if os.path.exists(stamp) else None
+    if have != build_oracle.sources_digest():
+        raise RuntimeError("oracle library %s was built from other sources than this tree's "
+                           "(run python oracle/build_oracle.py)" % path)
+
+
 def _load(path):
     if path not in _libs:
         if not os.path.exists(path):
             raise FileNotFoundError("oracle library not built: %s (run python oracle/build_oracle.py)" % path)
+        if path == LIB_PATH:
+            _check_fresh(path)
         _libs[path] = ctypes.CDLL(path)
     return _libs[path]
 

@@ -83,9 +83,31 @@ def our_models():
                   if os.path.exists(os.path.join(OUR_MODELS, d, "modelfunctions.h")))
 
 
+def sources_digest():
+    """SHA-1 over every source liboracle.so is compiled from (the oracle's
+    own headers and this project's plug-ins); bind.py refuses a library
+    whose stamp differs."""
+    import hashlib
+    h = hashlib.sha1()
+    files = [os.path.join(HERE, f) for f in sorted(os.listdir(HERE))
+             if f.endswith((".h", ".inc")) or f == "orc_common.cpp"]
+    for root, dirs, fs in sorted(os.walk(OUR_MODELS)):
+        dirs.sort()
+        files += [os.path.join(root, f) for f in sorted(fs)]
+    for f in files:
+        h.update(os.path.relpath(f, REPO).encode())
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
 def build_oracle():
     units = {m: ("orc_" + m, os.path.join(OUR_MODELS, m), "orc_%s_" % m) for m in our_models()}
-    return _build_lib(os.path.join(HERE, "_build"), "liboracle.so", units)
+    digest = sources_digest()
+    lib = _build_lib(os.path.join(HERE, "_build"), "liboracle.so", units)
+    with open(lib + ".sha", "w") as f:
+        f.write(digest + "\n")
+    return lib
 
 
 def build_ref():
