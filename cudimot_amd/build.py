@@ -92,7 +92,8 @@ DEFAULT_BATCHES = {"tensor_bingham2_sticks3": ["DiffusionTensor", "NODDI_Bingham
 
 PLUGIN_MACROS = ["MODEL_H_INCLUDED", "NPARAMS", "NCFP", "NFIXP", "CDM_LM_JACOBIAN_DOUBLE", "CDM_VOXEL_CACHE",
                  "CDM_VOXEL_CACHE_DEPENDS", "CDM_MEAS_CACHE", "CDM_NUMERICAL_MASK", "CDM_MEAS_VOXEL_CACHE",
-                 "CDM_MEAS_VOXEL_CACHE_DEPENDS", "CDM_VOXEL_CACHE_JOBS", "CDM_SIGNAL_PARTIALS"]
+                 "CDM_MEAS_VOXEL_CACHE_DEPENDS", "CDM_VOXEL_CACHE_JOBS", "CDM_VOXEL_CACHE_JOBS_DEPENDS",
+                 "CDM_MEAS_VOXEL_CACHE_FILL", "CDM_SIGNAL_PARTIALS"]
 
 
 def batch_unit_source(name, model_dirs):

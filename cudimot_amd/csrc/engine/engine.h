@@ -49,6 +49,15 @@ constexpr int cdm_ntri = NPARAMS * (NPARAMS + 1) / 2;
 //                              FixP, vc, mc, mvc) uses them.  MCMC keeps them
 //                              per voxel so that a step in another parameter
 //                              reuses them
+//   CDM_MEAS_VOXEL_CACHE_FILL  Predicted_Signal_MV_Fill(npar, P, CFP, FixP,
+//                              vc, mc, mvc) fills mvc as Meas_Voxel_Cache and
+//                              returns the signal of Predicted_Signal_MV,
+//                              sharing the work the two have in common
+//   CDM_VOXEL_CACHE_JOBS_DEPENDS b  the jobs of CDM_VOXEL_CACHE_JOBS depend
+//                              only on the parameters in b (a subset of
+//                              CDM_VOXEL_CACHE_DEPENDS): an MCMC step in
+//                              another cache parameter recombines the kept
+//                              job values instead of recomputing them
 //   CDM_NUMERICAL_MASK b       derivatives the engine forms by central
 //                              differences (the reference's NUMERICAL step
 //                              rule) with per-perturbation voxel caches;
@@ -88,8 +97,14 @@ __device__ __forceinline__ void cdm_prepare(TF* p, TF* fx, CDM_F64* vc) {
 
 #ifdef CDM_VOXEL_CACHE_JOBS
 constexpr int cdm_vc_jobs = CDM_VOXEL_CACHE_JOBS;
+#ifdef CDM_VOXEL_CACHE_JOBS_DEPENDS
+constexpr unsigned cdm_vcj_dep = CDM_VOXEL_CACHE_JOBS_DEPENDS;
+#else
+constexpr unsigned cdm_vcj_dep = cdm_vc_dep;
+#endif
 #else
 constexpr int cdm_vc_jobs = 0;
+constexpr unsigned cdm_vcj_dep = cdm_vc_dep;
 #endif
 
 // The voxel cache computed by a group of L lanes.  A plug-in that declares
@@ -97,20 +112,29 @@ constexpr int cdm_vc_jobs = 0;
 // (Voxel_Cache_Job) and a combination (Voxel_Cache_Combine); lane l computes
 // jobs l, l+L, ..., the group exchanges the results and every lane combines
 // them, with the arithmetic of Voxel_Cache.
+#if defined(CDM_VOXEL_CACHE) && defined(CDM_VOXEL_CACHE_JOBS)
+// The job values of the voxel cache at p, spread over the group's lanes
+// (every lane ends with all of them).
+template <int L, class TF>
+__device__ __forceinline__ void cdm_cache_jobs_g(TF* p, TF* fx, CDM_F64* jr, int lane) {
+  constexpr int rounds = (cdm_vc_jobs + L - 1) / L;
+  CDM_F64 mine[rounds];
+#pragma unroll
+  for (int r = 0; r < rounds; ++r) {
+    const int j = lane + r * L;
+    mine[r] = Voxel_Cache_Job(j < cdm_vc_jobs ? j : 0, cdm_np, p, fx);
+  }
+#pragma unroll
+  for (int j = 0; j < cdm_vc_jobs; ++j) jr[j] = cdm::Group<L>::bcast(mine[j / L], j % L);
+}
+#endif
+
 template <int L, class TF>
 __device__ __forceinline__ void cdm_prepare_g(TF* p, TF* fx, CDM_F64* vc, int lane) {
 #if defined(CDM_VOXEL_CACHE) && defined(CDM_VOXEL_CACHE_JOBS)
   if constexpr (L > 1) {
-    constexpr int rounds = (cdm_vc_jobs + L - 1) / L;
-    CDM_F64 mine[rounds];
-#pragma unroll
-    for (int r = 0; r < rounds; ++r) {
-      const int j = lane + r * L;
-      mine[r] = Voxel_Cache_Job(j < cdm_vc_jobs ? j : 0, cdm_np, p, fx);
-    }
     CDM_F64 jr[cdm_vc_jobs];
-#pragma unroll
-    for (int j = 0; j < cdm_vc_jobs; ++j) jr[j] = cdm::Group<L>::bcast(mine[j / L], j % L);
+    cdm_cache_jobs_g<L>(p, fx, jr, lane);
     Voxel_Cache_Combine(cdm_np, p, fx, jr, vc);
     return;
   }
@@ -979,15 +1003,21 @@ __device__ __forceinline__ double cdm_neg_loglik_mv(const cdm_geom& g, const flo
     cdm_tf* cfpm = crow.at(s_cfp, m, g.cfp_ts);
     const CDM_F64* mcm = cdm_mc_row(s_mc, m);
     cdm_tf mvc[cdm_mvc];
+    cdm_tf s;
     if constexpr (MODE == 2) {
+#ifdef CDM_MEAS_VOXEL_CACHE_FILL
+      s = Predicted_Signal_MV_Fill(cdm_np, pf, cfpm, fx, vc, mcm, mvc);
+#else
       Meas_Voxel_Cache(cdm_np, pf, cfpm, fx, vc, mcm, mvc);
+      s = Predicted_Signal_MV(cdm_np, pf, cfpm, fx, vc, mcm, mvc);
+#endif
 #pragma unroll
       for (int k = 0; k < cdm_mvc; ++k) row[m * cdm_mvc + k] = mvc[k];
     } else {
 #pragma unroll
       for (int k = 0; k < cdm_mvc; ++k) mvc[k] = row[m * cdm_mvc + k];
+      s = Predicted_Signal_MV(cdm_np, pf, cfpm, fx, vc, mcm, mvc);
     }
-    const cdm_tf s = Predicted_Signal_MV(cdm_np, pf, cfpm, fx, vc, mcm, mvc);
     if constexpr (RICIAN) {
       const double y = mv[m], sd = (double)s;
       acc += log(y) + (-0.5 * tau * (y * y + sd * sd) + cdm_logI0(tau * sd * y));
@@ -1135,9 +1165,28 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
     tau_rej = st->tau_rej;
   }
 
-  // voxel cache of the current state (a deterministic function of p)
+  // voxel cache of the current state (a deterministic function of p); with
+  // a job-dependency mask narrower than the cache's, the current job values
+  // are kept too, so that a step in a parameter the jobs do not depend on
+  // only recombines them
+#if defined(CDM_VOXEL_CACHE) && defined(CDM_VOXEL_CACHE_JOBS)
+  constexpr bool job_split = L > 1 && cdm_vcj_dep != cdm_vc_dep;
+#else
+  constexpr bool job_split = false;
+#endif
   CDM_F64 vc_cur[cdm_vc];
-  cdm_prepare_dg<L>(p, fx, vc_cur, lane);
+  CDM_F64 jr_cur[job_split ? cdm_vc_jobs : 1];
+  if constexpr (job_split) {
+#if defined(CDM_VOXEL_CACHE) && defined(CDM_VOXEL_CACHE_JOBS)
+    cdm_tf pf[cdm_np];
+#pragma unroll
+    for (int k = 0; k < cdm_np; ++k) pf[k] = (cdm_tf)p[k];
+    cdm_cache_jobs_g<L>(pf, fx, jr_cur, lane);
+    Voxel_Cache_Combine(cdm_np, pf, fx, jr_cur, vc_cur);
+#endif
+  } else {
+    cdm_prepare_dg<L>(p, fx, vc_cur, lane);
+  }
   // per-measurement caches: two rows per voxel, `cur` holds the current state's
   cdm_tf* mvc_rows = cdm_mvc ? a.mvc + (size_t)v * 2 * a.g.nmeas * cdm_mvc : nullptr;
   int cur = 0;
@@ -1183,8 +1232,27 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
         // the voxel cache changes only with the parameters it depends on
         const bool dep = (cdm_vc_dep >> i) & 1u;
         const bool mdep = (cdm_mvc_dep >> i) & 1u;
+        const bool jdep = (cdm_vcj_dep >> i) & 1u;
         CDM_F64 vc_new[cdm_vc];
-        if (dep) cdm_prepare_dg<L>(p, fx, vc_new, lane);
+        CDM_F64 jr_new[job_split ? cdm_vc_jobs : 1];
+        if (dep) {
+          if constexpr (job_split) {
+#if defined(CDM_VOXEL_CACHE) && defined(CDM_VOXEL_CACHE_JOBS)
+            cdm_tf pf[cdm_np];
+#pragma unroll
+            for (int k = 0; k < cdm_np; ++k) pf[k] = (cdm_tf)p[k];
+            if (jdep) {
+              cdm_cache_jobs_g<L>(pf, fx, jr_new, lane);
+            } else {
+#pragma unroll
+              for (int j = 0; j < cdm_vc_jobs; ++j) jr_new[j] = jr_cur[j];
+            }
+            Voxel_Cache_Combine(cdm_np, pf, fx, jr_new, vc_new);
+#endif
+          } else {
+            cdm_prepare_dg<L>(p, fx, vc_new, lane);
+          }
+        }
         // element-wise select: a pointer select between two register arrays
         // would place both in scratch memory
         CDM_F64 vcu[cdm_vc];
@@ -1211,6 +1279,10 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
           energy = e_new;
 #pragma unroll
           for (int k = 0; k < cdm_vc; ++k) vc_cur[k] = vcu[k];
+          if constexpr (job_split) {
+#pragma unroll
+            for (int j = 0; j < cdm_vc_jobs; ++j) jr_cur[j] = (dep && jdep) ? jr_new[j] : jr_cur[j];
+          }
           if (mdep) cur = 1 - cur;
         } else {
           ++nrej[i];

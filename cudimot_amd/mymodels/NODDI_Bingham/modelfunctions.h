@@ -11,15 +11,19 @@
 // only, so they are computed once per parameter vector (Voxel_Cache) instead
 // of once per measurement; the intra-axonal signal of each measurement does
 // not depend on fiso or fintra, so an MCMC step in those two reuses it
-// (Meas_Voxel_Cache, Predicted_Signal_MV).
+// (Meas_Voxel_Cache, Predicted_Signal_MV; Predicted_Signal_MV_Fill does both
+// from one evaluation of the local gradient); the five 1F1 evaluations
+// depend on kappa and beta only, so an MCMC step in fintra recombines them.
 #include "bingham_math.h"
 
 #define CDM_VOXEL_CACHE 4
 #define CDM_VOXEL_CACHE_DEPENDS 0x0Eu       // fintra, kappa, beta
 #define CDM_VOXEL_CACHE_JOBS 5              // its five 1F1 evaluations
+#define CDM_VOXEL_CACHE_JOBS_DEPENDS 0x0Cu  // kappa, beta
 #define CDM_NUMERICAL_MASK 0x7Cu            // kappa, beta, th, ph, psi
 #define CDM_MEAS_VOXEL_CACHE 1              // intra-axonal signal per measurement
 #define CDM_MEAS_VOXEL_CACHE_DEPENDS 0x7Cu  // kappa, beta, th, ph, psi
+#define CDM_MEAS_VOXEL_CACHE_FILL 1         // cache row and signal together
 #define CDM_SIGNAL_PARTIALS 1               // signal and analytic entries together
 #define CDM_MEAS_CACHE 1                    // exp(-b Diso) per measurement
 
@@ -82,6 +86,15 @@ MACRO T Predicted_Signal_MV(int npar, T* P, T* CFP, T* FixP, const CDM_F64* vc, 
   f.dperp2 = vc[3];
   const T e = bingham_fan_extra(P[4], P[5], P[6], f, CFP);
   const T fan = ((T)1 - P[1]) * e + P[1] * mvc[0];
+  const T iso = (T)mc[0];
+  return FixP[0] * (((T)1 - P[0]) * fan + P[0] * iso);
+}
+
+// Meas_Voxel_Cache and Predicted_Signal_MV from one local gradient.
+MACRO T Predicted_Signal_MV_Fill(int npar, T* P, T* CFP, T* FixP, const CDM_F64* vc, const CDM_F64* mc, T* mvc) {
+  T fan, e, i;
+  bingham1_parts(P, CFP, vc, fan, e, i);
+  mvc[0] = i;
   const T iso = (T)mc[0];
   return FixP[0] * (((T)1 - P[0]) * fan + P[0] * iso);
 }
