@@ -107,11 +107,6 @@ constexpr int cdm_vc_jobs = 0;
 constexpr unsigned cdm_vcj_dep = cdm_vc_dep;
 #endif
 
-// The voxel cache computed by a group of L lanes.  A plug-in that declares
-// CDM_VOXEL_CACHE_JOBS n splits Voxel_Cache into n independent scalar jobs
-// (Voxel_Cache_Job) and a combination (Voxel_Cache_Combine); lane l computes
-// jobs l, l+L, ..., the group exchanges the results and every lane combines
-// them, with the arithmetic of Voxel_Cache.
 #if defined(CDM_VOXEL_CACHE) && defined(CDM_VOXEL_CACHE_JOBS)
 // The job values of the voxel cache at p, spread over the group's lanes
 // (every lane ends with all of them).
@@ -129,6 +124,11 @@ __device__ __forceinline__ void cdm_cache_jobs_g(TF* p, TF* fx, CDM_F64* jr, int
 }
 #endif
 
+// The voxel cache computed by a group of L lanes.  A plug-in that declares
+// CDM_VOXEL_CACHE_JOBS n splits Voxel_Cache into n independent scalar jobs
+// (Voxel_Cache_Job) and a combination (Voxel_Cache_Combine); lane l computes
+// jobs l, l+L, ..., the group exchanges the results and every lane combines
+// them, with the arithmetic of Voxel_Cache.
 template <int L, class TF>
 __device__ __forceinline__ void cdm_prepare_g(TF* p, TF* fx, CDM_F64* vc, int lane) {
 #if defined(CDM_VOXEL_CACHE) && defined(CDM_VOXEL_CACHE_JOBS)
