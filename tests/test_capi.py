@@ -67,6 +67,21 @@ def test_argument_errors_without_gpu():
     assert "FixP" in L.cdm_last_error().decode()
 
 
+def test_empty_volume_is_a_no_op_without_gpu():
+    """Zero voxels: every method returns success before touching the device
+    (a part of a split volume may be empty); the data pointers may be null."""
+    m = capi.Model("NODDI_Bingham")
+    L = capi.lib()
+    cfp = 8  # never dereferenced
+    assert L.cdm_levmar(m.h, 0, 99, None, None, cfp, None, 200, 1, None, None, None) == 0
+    assert L.cdm_mcmc(m.h, 0, 99, None, None, cfp, None, 10, 10, 5, 1, 0, 1, 0, None, None, None) == 0
+    assert L.cdm_predicted_signal(m.h, 0, 99, 4, None, cfp, None, None, None) == 0
+    assert L.cdm_bic_aic(m.h, 0, 99, 4, 0, None, None, None, cfp, None, None, None, None) == 0
+    # the size checks still apply
+    assert L.cdm_levmar(m.h, -1, 99, None, None, cfp, None, 200, 1, None, None, None) != 0
+    assert L.cdm_mcmc(m.h, 0, 99, None, None, cfp, None, 10, 4, 5, 1, 0, 1, 0, None, None, None) != 0
+
+
 def test_rician_clean_matches_reference_rule():
     """dMRI_Data.cu:14-31: non-positive entries take the smallest positive
     value when some entries are positive; all-non-positive voxels stay."""

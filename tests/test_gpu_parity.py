@@ -351,3 +351,30 @@ def test_bic_aic_parity(torch_cuda, model, rician):
     for got, ref in ((H.host(bic), b_ref), (H.host(aic), a_ref)):
         err = np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)
         assert err.max() <= 1e-4, (model, rician, err.max())
+
+
+def test_empty_volume_all_methods(torch_cuda):
+    """Zero voxels through every method (GridSearch included, which stages its
+    grid first): success, and the next launch on the same handle is unaffected."""
+    from cudimot_amd import capi, synth
+    m = capi.Model("Ball_1_Stick")
+    cfp = H.dev(synth.model_acquisition("Ball_1_Stick"))
+    M = cfp.shape[0]
+    gp, grid = m.load_grid(GRIDS["Ball_1_Stick"], is_text=True)
+    e_meas = torch_cuda.empty((0, M), device="cuda")
+    e_par = torch_cuda.empty((0, 5), device="cuda")
+    m.gridsearch(e_meas, e_par, cfp)
+    m.levmar(e_meas, e_par, cfp)
+    m.mcmc(e_meas, e_par, cfp, None, torch_cuda.empty((0, 5, 2), device="cuda"), None, nburn=4, njumps=4,
+           sampleevery=2)
+    m.predicted_signal(torch_cuda.empty((0, 5, 2), device="cuda"), cfp, None, torch_cuda.empty((0, M), device="cuda"))
+    m.bic_aic(e_meas, torch_cuda.empty((0, 5, 2), device="cuda"), cfp, None, torch_cuda.empty(0, device="cuda"),
+              torch_cuda.empty(0, device="cuda"))
+    capi.sync()
+    # a real launch afterwards
+    o = bind.OracleModel("Ball_1_Stick")
+    _, t, _, noisy = H.volume(o, "Ball_1_Stick", 64, seed=5)
+    out = torch_cuda.zeros((64, M), dtype=torch_cuda.float32, device="cuda")
+    m.predicted_signal(H.dev(t.reshape(64, -1, 1)), cfp, None, out)
+    capi.sync()
+    assert np.all(np.isfinite(H.host(out)))
