@@ -149,7 +149,8 @@ def batch_unit_source(name, model_dirs):
     for k in range(len(model_dirs)):
         out += ["      case %d: {" % k,
                 "        if (b.g.cfp_ts > %s::cdm_cfp_max || b.g.fixp_ts > %s::cdm_fixp_max) return -1;" % (ns(k), ns(k)),
-                "        need = mode == cdm::kBatchMcmc ? %s::op_lds_mcmc(b.g, b.L) : %s::op_lds(b.g);" % (ns(k), ns(k)),
+                "        need = mode == cdm::kBatchMcmc ? %s::op_lds_mcmc(b.g, b.L) : %s::op_lds_lm(b.g, b.L);" % (ns(k), ns(k)),
+                "        if (need > %s::cdm_lds_max) return -1;" % ns(k),
                 "        if (mode == cdm::kBatchLevMar) {",
                 "          const auto a = %s::op_lm_args(b.g, b.params, b.max_iters, b.c, b.iters, b.cost);" % ns(k),
                 "          std::memcpy(d.args, &a, sizeof(a));", "        } else {",

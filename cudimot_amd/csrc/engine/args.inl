@@ -37,6 +37,16 @@ static size_t op_lds(const cdm::Geom& g) {
   return (size_t)cdm_mc_offset(g.nmeas, g.cfp_ts) * sizeof(float) + (size_t)g.nmeas * cdm_mc * sizeof(CDM_F64);
 }
 
+// LDS of a CU (gfx950), the most one block can have.
+constexpr size_t cdm_lds_max = 160 * 1024;
+
+// LM launches: the 64-lane groups' J^T J scratch after the shared tables.
+static size_t op_lds_lm(const cdm::Geom& g, int L) {
+  const size_t extra = cdm_lm_scratch_doubles(L, g.nmeas);
+  if (!extra) return op_lds(g);
+  return cdm_lm_scratch_offset(g.nmeas, g.cfp_ts) + (size_t)(cdm_block / L) * extra * sizeof(CDM_F64);
+}
+
 static dim3 op_grid(int nvox, int L) {
   const int per_block = cdm_block / L;
   return dim3((unsigned)((nvox + per_block - 1) / per_block));

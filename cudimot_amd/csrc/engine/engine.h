@@ -668,6 +668,58 @@ struct cdm_jacobian_setup {
   }
 };
 
+// A 64-lane LM group (one whole wave) sums J^T J and J^T r through LDS: each
+// lane stores the rows of its measurements, then lane e sums entry e (a
+// packed lower-triangle element of J^T J, then the elements of J^T r) over
+// all measurements in order, in FP64.  The per-lane accumulators of the
+// other group sizes (P(P+1)/2 + P values live across every measurement's
+// evaluation) are then never live with the Jacobian's evaluations, which
+// with many parameters outgrow the register file.  The per-group scratch
+// sits after the CFP table and the measurement cache in the dynamic LDS.
+template <int L>
+constexpr bool cdm_lm_lds_normal = (L == 64);
+
+__host__ __device__ __forceinline__ size_t cdm_lm_scratch_offset(int nmeas, int cfp_ts) {
+  const size_t b = (size_t)cdm_mc_offset(nmeas, cfp_ts) * sizeof(float) + (size_t)nmeas * cdm_mc * sizeof(CDM_F64);
+  return (b + 15) / 16 * 16;
+}
+
+// Doubles of LDS scratch per LM voxel group: the [M][P+1] rows (J, r), then
+// the P(P+1)/2 + P sums; none below 64 lanes.
+__host__ __device__ __forceinline__ size_t cdm_lm_scratch_doubles(int L, int nmeas) {
+  return L == 64 ? (size_t)nmeas * (cdm_np + 1) + cdm_ntri + cdm_np : 0;
+}
+
+// LDS stores of one wave made visible to the wave's other lanes.
+__device__ __forceinline__ void cdm_wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// sums[e] for e < P(P+1)/2: sum_m J_mi J_mj of packed element e = i(i+1)/2 + j;
+// sums[P(P+1)/2 + i]: sum_m J_mi r_m.  rows: [nmeas][P+1].
+template <int L>
+__device__ __forceinline__ void cdm_lds_normal_sums(const CDM_F64* rows, CDM_F64* sums, int nmeas, int lane) {
+  constexpr int T = cdm_ntri + cdm_np;
+  for (int e = lane; e < T; e += L) {
+    int i = 0, j = 0;
+    if (e < cdm_ntri) {
+      int k = 0;
+#pragma unroll
+      for (int ii = 0; ii < cdm_np; ++ii)
+        if (e >= k + ii + 1) k += ii + 1;
+        else if (e >= k) { i = ii; j = e - k; k = cdm_ntri + 1; }
+    } else {
+      i = e - cdm_ntri;
+      j = cdm_np;
+    }
+    double acc = 0.0;
+    for (int m = 0; m < nmeas; ++m) acc += rows[m * (cdm_np + 1) + i] * rows[m * (cdm_np + 1) + j];
+    sums[e] = acc;
+  }
+}
+
 // The kernel bodies take the block index as an argument so that the batched
 // persistent kernel (one launch over several models' voxels) can run them on
 // tiles it takes from a work queue.  A body stages the CFP table (two block
@@ -719,33 +771,58 @@ __device__ __forceinline__ void cdm_levmar_voxel(const cdm_lm_args& a, int marqu
       cdm_cfp_row<cdm_jt> rowj;
       cdm_jacobian_setup<cdm_jt> jac;
       jac.template init<L>(pj, fxj, lane);
-      for (int m = lane; m < a.g.nmeas; m += L) {
-        cdm_jt* cm = rowj.at(s_cfp, m, a.g.cfp_ts);
-        cdm_jt d[cdm_np];
-        const cdm_jt r = jac.eval(pj, cm, fxj, cdm_mc_row(s_mc, m), d) - (cdm_jt)mv[m];
-        cdm_lm_acc df[cdm_np];
+      if constexpr (cdm_lm_lds_normal<L>) {
+        CDM_F64* rows = reinterpret_cast<CDM_F64*>(reinterpret_cast<char*>(s_cfp) +
+                                                   cdm_lm_scratch_offset(a.g.nmeas, a.g.cfp_ts)) +
+                        (size_t)(threadIdx.x / L) * cdm_lm_scratch_doubles(L, a.g.nmeas);
+        CDM_F64* sums = rows + (size_t)a.g.nmeas * (cdm_np + 1);
+        for (int m = lane; m < a.g.nmeas; m += L) {
+          cdm_jt* cm = rowj.at(s_cfp, m, a.g.cfp_ts);
+          cdm_jt d[cdm_np];
+          const cdm_jt r = jac.eval(pj, cm, fxj, cdm_mc_row(s_mc, m), d) - (cdm_jt)mv[m];
 #pragma unroll
-        for (int i = 0; i < cdm_np; ++i) df[i] = (cdm_lm_acc)(d[i] * dtf[i]);
-        const cdm_lm_acc rf = (cdm_lm_acc)r;
+          for (int i = 0; i < cdm_np; ++i) rows[m * (cdm_np + 1) + i] = (CDM_F64)(cdm_lm_acc)(d[i] * dtf[i]);
+          rows[m * (cdm_np + 1) + cdm_np] = (CDM_F64)(cdm_lm_acc)r;
+        }
+        cdm_wave_lds_sync();
+        cdm_lds_normal_sums<L>(rows, sums, a.g.nmeas, lane);
+        cdm_wave_lds_sync();
 #pragma unroll
-        for (int i = 0; i < cdm_np; ++i) gl[i] += rf * df[i];
-        int k = 0;
+        for (int i = 0; i < cdm_np; ++i) gl[i] = (cdm_lm_acc)sums[cdm_ntri + i];
 #pragma unroll
-        for (int i = 0; i < cdm_np; ++i)
+        for (int k = 0; k < cdm_ntri; ++k) hs[k] = (cdm_lm_acc)sums[k];
+        // the next iteration's stores into `rows` must not pass this one's reads
+        cdm_wave_lds_sync();
+      } else {
+        for (int m = lane; m < a.g.nmeas; m += L) {
+          cdm_jt* cm = rowj.at(s_cfp, m, a.g.cfp_ts);
+          cdm_jt d[cdm_np];
+          const cdm_jt r = jac.eval(pj, cm, fxj, cdm_mc_row(s_mc, m), d) - (cdm_jt)mv[m];
+          cdm_lm_acc df[cdm_np];
 #pragma unroll
-          for (int j = 0; j <= i; ++j) hs[k++] += df[i] * df[j];
+          for (int i = 0; i < cdm_np; ++i) df[i] = (cdm_lm_acc)(d[i] * dtf[i]);
+          const cdm_lm_acc rf = (cdm_lm_acc)r;
+#pragma unroll
+          for (int i = 0; i < cdm_np; ++i) gl[i] += rf * df[i];
+          int k = 0;
+#pragma unroll
+          for (int i = 0; i < cdm_np; ++i)
+#pragma unroll
+            for (int j = 0; j <= i; ++j) hs[k++] += df[i] * df[j];
+        }
+#pragma unroll
+        for (int i = 0; i < cdm_np; ++i) gl[i] = cdm::Group<L>::sum(gl[i]);
+#pragma unroll
+        for (int k = 0; k < cdm_ntri; ++k) hs[k] = cdm::Group<L>::sum(hs[k]);
       }
 #pragma unroll
-      for (int i = 0; i < cdm_np; ++i) {
-        gl[i] = cdm::Group<L>::sum(gl[i]);
-        grad[i] = a.c.fixed[i] ? 0.0 : 2.0 * (double)gl[i];
-      }
+      for (int i = 0; i < cdm_np; ++i) grad[i] = a.c.fixed[i] ? 0.0 : 2.0 * (double)gl[i];
       int k = 0;
 #pragma unroll
       for (int i = 0; i < cdm_np; ++i)
 #pragma unroll
         for (int j = 0; j <= i; ++j, ++k) {
-          cdm_lm_acc h = (cdm_lm_acc)2 * cdm::Group<L>::sum(hs[k]);
+          cdm_lm_acc h = (cdm_lm_acc)2 * hs[k];
           if (a.c.fixed[i] || a.c.fixed[j]) h = (i == j) ? (cdm_lm_acc)1 : (cdm_lm_acc)0;
           hs[k] = h;
         }

@@ -73,15 +73,17 @@ static hipError_t op_levmar(int L, int W, const cdm::Geom& g, float* params, int
                             const cdm::Consts& c, int* iters, float* cost, int* queue, int ncu, hipStream_t s) {
   if (g.nvox <= 0) return hipSuccess;
   if (g.cfp_ts > cdm_cfp_max || g.fixp_ts > cdm_fixp_max) return hipErrorInvalidValue;
+  // 64-lane groups keep the rows of J in LDS; past the LDS of a CU, 16 lanes
+  if (L == 64 && op_lds_lm(g, 64) > cdm_lds_max) L = 16;
   cdm_lm_args a = op_lm_args(g, params, max_iters, c, iters, cost);
   if (!queue) {
     CDM_DISPATCH_L(L, CDM_DISPATCH_W(W, hipLaunchKernelGGL((cdm_levmar_kernel<LG, WG>), op_grid(g.nvox, LG),
-                                                           dim3(cdm_block), op_lds(g), s, a, marquardt)));
+                                                           dim3(cdm_block), op_lds_lm(g, LG), s, a, marquardt)));
     return hipGetLastError();
   }
   a.queue = queue;
   hipLaunchKernelGGL(cdm_queue_reset_kernel, dim3(1), dim3(1), 0, s, queue);
-  const size_t lds = op_lds(g);
+  const size_t lds = op_lds_lm(g, L);
   CDM_DISPATCH_L(L, CDM_DISPATCH_W(W, (op_levmar_queue_launch<LG, WG>(a, marquardt, g.nvox, lds, ncu, s))));
   return hipGetLastError();
 }
