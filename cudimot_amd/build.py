@@ -93,7 +93,7 @@ DEFAULT_BATCHES = {"tensor_bingham2_sticks3": ["DiffusionTensor", "NODDI_Bingham
 PLUGIN_MACROS = ["MODEL_H_INCLUDED", "NPARAMS", "NCFP", "NFIXP", "CDM_LM_JACOBIAN_DOUBLE", "CDM_VOXEL_CACHE",
                  "CDM_VOXEL_CACHE_DEPENDS", "CDM_MEAS_CACHE", "CDM_NUMERICAL_MASK", "CDM_MEAS_VOXEL_CACHE",
                  "CDM_MEAS_VOXEL_CACHE_DEPENDS", "CDM_VOXEL_CACHE_JOBS", "CDM_VOXEL_CACHE_JOBS_DEPENDS",
-                 "CDM_MEAS_VOXEL_CACHE_FILL", "CDM_SIGNAL_PARTIALS"]
+                 "CDM_MEAS_VOXEL_CACHE_FILL", "CDM_SIGNAL_PARTIALS", "CDM_WATSON_REFERENCE_FORMS"]
 
 
 def batch_unit_source(name, model_dirs):
@@ -102,7 +102,8 @@ def batch_unit_source(name, model_dirs):
     L = [1, 4, 16, 64]
     out = ["// generated: batch unit %s (%s)" % (name, ", ".join(m for m, _ in model_dirs)),
            '#include "abi/functions_gpu.h"', '#include "engine/philox.h"', '#include "engine/wave.h"',
-           '#include "host/model_ops.h"', "#include <algorithm>", "#include <climits>", "#include <cstring>", ""]
+           '#include "engine/packed.h"', '#include "host/model_ops.h"', "#include <algorithm>", "#include <climits>",
+           "#include <cstring>", ""]
     for k, (m, d) in enumerate(model_dirs):
         out += ["#undef %s" % x for x in PLUGIN_MACROS]
         out += ["namespace cdm_batch_%s_m%d {" % (name, k),

@@ -32,7 +32,7 @@ typedef cdm_tf cdm_jt;
 typedef float cdm_lm_acc;
 #endif
 constexpr int cdm_np = NPARAMS;
-constexpr int cdm_ntri = NPARAMS * (NPARAMS + 1) / 2;
+constexpr int cdm_ntri = cdm::tri_size(NPARAMS);
 
 // Optional invariant caches declared by the plug-in (engine extension, no
 // counterpart in the reference's plug-in API):
@@ -699,17 +699,15 @@ __device__ __forceinline__ void cdm_wave_lds_sync() {
 
 // sums[e] for e < P(P+1)/2: sum_m J_mi J_mj of packed element e = i(i+1)/2 + j;
 // sums[P(P+1)/2 + i]: sum_m J_mi r_m.  rows: [nmeas][P+1].
+static_assert(cdm::tri_ij_exact<cdm_np>(), "packed lower-triangle index");
+
 template <int L>
 __device__ __forceinline__ void cdm_lds_normal_sums(const CDM_F64* rows, CDM_F64* sums, int nmeas, int lane) {
   constexpr int T = cdm_ntri + cdm_np;
   for (int e = lane; e < T; e += L) {
     int i = 0, j = 0;
     if (e < cdm_ntri) {
-      int k = 0;
-#pragma unroll
-      for (int ii = 0; ii < cdm_np; ++ii)
-        if (e >= k + ii + 1) k += ii + 1;
-        else if (e >= k) { i = ii; j = e - k; k = cdm_ntri + 1; }
+      cdm::tri_ij<cdm_np>(e, i, j);
     } else {
       i = e - cdm_ntri;
       j = cdm_np;

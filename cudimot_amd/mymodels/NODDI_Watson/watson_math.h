@@ -97,14 +97,18 @@ __device__ inline void noddi_watson_sh_quad(CDM_F64 k, CDM_F64* c) {
 // expansion at or below 0.4.  Deviation from the reference: in
 // 0.4 < kappa <= 3 the closed forms cancel catastrophically (the reference's
 // intra-cellular signal is off by up to ~1% at kappa = 0.41, and by more
-// than 1e-6 up to kappa ~ 2), so that band uses noddi_watson_sh_quad.
+// than 1e-6 up to kappa ~ 2), so that band uses noddi_watson_sh_quad.  With
+// CDM_WATSON_REFERENCE_FORMS defined (the NODDI_Watson_refnum plug-in) the
+// reference's closed forms are kept in that band too.
 __device__ inline void noddi_watson_sh(CDM_F64 k, CDM_F64* c) {
   const CDM_F64 pi = 3.14159265358979323846;
   c[0] = 2.0 * sqrt(pi);
+#ifndef CDM_WATSON_REFERENCE_FORMS
   if (k > 0.4 && k <= 3.0) {
     noddi_watson_sh_quad(k, c);
     return;
   }
+#endif
   if (k > 30) {
     // fits in u = log(kappa) - log(30), coefficients of u^0..u^6
     const CDM_F64 fit[6][7] = {
@@ -182,12 +186,18 @@ __device__ inline void noddi_watson_sh(CDM_F64 k, CDM_F64* c) {
 // small x.  Deviation from the reference: it takes erf in single precision
 // (erff) inside its double path, and the upward recurrence multiplies that
 // error ~60-fold by I_6 at b = 700 s/mm^2, enough to move the intra-cellular
-// signal by up to ~1e-3 at large kappa; here erf is double.
+// signal by up to ~1e-3 at large kappa; here erf is double, except with
+// CDM_WATSON_REFERENCE_FORMS (the NODDI_Watson_refnum plug-in), which takes
+// the reference's single-precision erf of the float-rounded argument.
 __device__ inline void noddi_lgi(CDM_F64 x, CDM_F64* L) {
   if (x > 0.05) {
     CDM_F64 I[7];
     const CDM_F64 sx = sqrt(x);
+#ifdef CDM_WATSON_REFERENCE_FORMS
+    I[0] = sqrt(3.14159265358979323846) * (CDM_F64)erff((float)sx) / sx;
+#else
     I[0] = sqrt(3.14159265358979323846) * erf(sx) / sx;
+#endif
     const CDM_F64 dx = 1.0 / x, emx = -exp(-x);
     for (int i = 1; i < 7; ++i) {
       I[i] = emx + ((i + 1) - 1.5) * I[i - 1];

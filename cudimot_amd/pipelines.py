@@ -107,22 +107,26 @@ def bingham_gridsearch_lm(meas, cfp, s0, lm_iters=200, grid=BINGHAM_GRID, frame=
     return p, cost
 
 
-def watson_gridsearch_lm(meas, cfp, s0, lm_iters=200, grid=WATSON_GRID, frame=None, kept=None, stream=None):
+def watson_gridsearch_lm(meas, cfp, s0, lm_iters=200, grid=WATSON_GRID, frame=None, kept=None, stream=None,
+                         model="NODDI_Watson"):
     """GridSearch + LM recipe of the reference's Watson pipeline; returns
     (params [n][5], cost [n]) on the device.  kept (device int [n],
-    optional) receives the GridSearch combination per voxel."""
+    optional) receives the GridSearch combination per voxel.  model
+    "NODDI_Watson_refnum" runs the recipe with the reference's own numerical
+    forms (closed-form SH coefficients for 0.4 < kappa <= 3, single-precision
+    erf in the Legendre-Gaussian integrals)."""
     torch = _torch()
     n = meas.shape[0]
     d_cfp = _dev(cfp)
     if frame is None:
         frame = tensor_frame(meas, cfp, stream=stream)
-    spec = capi.Model("NODDI_Watson").spec()
+    spec = capi.Model(model).spec()
     p = _dev(np.tile(np.asarray(spec["p_init"], dtype=np.float32), (n, 1)))
     p[:, 3:5] = frame[:, 0:2]
-    _model("NODDI_Watson", fixed="3,4", grid=grid).gridsearch(meas, p, d_cfp, s0, kept=kept, stream=stream)
-    _model("NODDI_Watson", fixed="2,3,4").levmar(meas, p, d_cfp, s0, max_iters=lm_iters, stream=stream)
+    _model(model, fixed="3,4", grid=grid).gridsearch(meas, p, d_cfp, s0, kept=kept, stream=stream)
+    _model(model, fixed="2,3,4").levmar(meas, p, d_cfp, s0, max_iters=lm_iters, stream=stream)
     cost = torch.empty(n, device=meas.device)
-    capi.Model("NODDI_Watson").levmar(meas, p, d_cfp, s0, max_iters=lm_iters, cost=cost, stream=stream)
+    capi.Model(model).levmar(meas, p, d_cfp, s0, max_iters=lm_iters, cost=cost, stream=stream)
     return p, cost
 
 

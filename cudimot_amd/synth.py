@@ -127,7 +127,8 @@ ACQUISITIONS = {
 }
 
 MODEL_ACQUISITION = {"Ball_1_Stick": "M64", "Ball_2_Sticks": "M280", "Ball_3_Sticks": "M280",
-                     "DiffusionTensor": "M64", "NODDI_Watson": "M99", "NODDI_Bingham": "M99",
+                     "DiffusionTensor": "M64", "NODDI_Watson": "M99", "NODDI_Watson_refnum": "M99",
+                     "NODDI_Bingham": "M99",
                      "NODDI_Bingham_2Fans": "M99", "toymodel": None}
 
 
@@ -150,7 +151,7 @@ def clip_to_bounds(model, t):
         for k, b in ((3, 4), (9, 10)):
             t[:, k] = np.clip(t[:, k], 0.2, 63.0)
             t[:, b] = np.clip(np.minimum(t[:, b], t[:, k] - 0.6), 0.02, 31.0)
-    elif model == "NODDI_Watson":
+    elif model in ("NODDI_Watson", "NODDI_Watson_refnum"):
         t[:, 2] = np.clip(t[:, 2], 0.01, 63.0)
     return t
 
@@ -160,6 +161,7 @@ def truth_for(model, n, seed):
            "Ball_2_Sticks": lambda n, s: truth_ball_k_sticks(n, 2, s),
            "Ball_3_Sticks": lambda n, s: truth_ball_k_sticks(n, 3, s),
            "DiffusionTensor": truth_dti, "NODDI_Watson": truth_noddi_watson,
+           "NODDI_Watson_refnum": truth_noddi_watson,
            "NODDI_Bingham": truth_noddi_bingham, "NODDI_Bingham_2Fans": truth_noddi_bingham_2fans,
            "toymodel": truth_toymodel}[model]
     return clip_to_bounds(model, gen(n, seed))

@@ -76,6 +76,11 @@ def philox4x32_10(ctr, key):
     return [int(x) for x in o]
 
 
+def omp_threads():
+    """Threads the oracle's OpenMP loops run on (the team actually formed)."""
+    return int(_load(LIB_PATH).orc_omp_threads())
+
+
 def draws(seed, voxel, jump, slot):
     lib = _load(LIB_PATH)
     z = ctypes.c_double()
@@ -83,6 +88,14 @@ def draws(seed, voxel, jump, slot):
     lib.orc_draws(ctypes.c_ulonglong(seed), ctypes.c_ulonglong(voxel), ctypes.c_uint32(jump),
                   ctypes.c_uint32(slot), ctypes.byref(z), ctypes.byref(u))
     return z.value, u.value
+
+
+def _seconds_ptr(seconds, nvox):
+    """Per-voxel wall-clock output of a fit (a float64 array of nvox, or None)."""
+    if seconds is None:
+        return None
+    assert seconds.dtype == np.float64 and seconds.flags.c_contiguous and seconds.size >= nvox
+    return seconds.ctypes.data_as(_D)
 
 
 class OracleModel:
@@ -138,7 +151,7 @@ class OracleModel:
         self._fn("constraints_mcmc")(p.shape[0], pp, ok.ctypes.data_as(_I))
         return ok.astype(bool)
 
-    def gridsearch(self, spec, meas, params, cfp, fixp, grid_params, grid):
+    def gridsearch(self, spec, meas, params, cfp, fixp, grid_params, grid, seconds=None):
         nvox, nmeas = meas.shape
         m, mp = _d(meas)
         p = np.array(params, dtype=np.float64, order="C")
@@ -152,10 +165,10 @@ class OracleModel:
         kept = np.zeros(nvox, dtype=np.int32)
         ncombs = g.shape[0] if len(gp) else 0
         self._fn("gridsearch")(btp, lop, hip, nvox, nmeas, mp, p.ctypes.data_as(_D), cp, xp, len(gp), gpp,
-                               ncombs, gptr, kept.ctypes.data_as(_I))
+                               ncombs, gptr, kept.ctypes.data_as(_I), _seconds_ptr(seconds, nvox))
         return p, kept
 
-    def levmar(self, spec, meas, params, cfp, fixp=None, max_iters=200, marquardt=True):
+    def levmar(self, spec, meas, params, cfp, fixp=None, max_iters=200, marquardt=True, seconds=None):
         nvox, nmeas = meas.shape
         m, mp = _d(meas)
         p = np.array(params, dtype=np.float64, order="C")
@@ -168,11 +181,12 @@ class OracleModel:
         iters = np.zeros(nvox, dtype=np.int32)
         cost = np.zeros(nvox)
         self._fn("levmar")(btp, lop, hip, fxp, nvox, nmeas, mp, p.ctypes.data_as(_D), cp, xp, int(max_iters),
-                           int(bool(marquardt)), iters.ctypes.data_as(_I), cost.ctypes.data_as(_D))
+                           int(bool(marquardt)), iters.ctypes.data_as(_I), cost.ctypes.data_as(_D),
+                           _seconds_ptr(seconds, nvox))
         return p, iters, cost
 
     def mcmc(self, spec, meas, params, cfp, fixp=None, nburn=1000, njumps=1250, sampleevery=25,
-             update_prop=True, rician=False, seed=8219, voxel_offset=0):
+             update_prop=True, rician=False, seed=8219, voxel_offset=0, seconds=None):
         nvox, nmeas = meas.shape
         m, mp = _d(meas)
         p = np.array(params, dtype=np.float64, order="C")
@@ -186,7 +200,7 @@ class OracleModel:
         self._fn("mcmc")(*[a[1] for a in arrs], nvox, nmeas, mp, p.ctypes.data_as(_D), cp, xp, int(nburn),
                          int(njumps), int(sampleevery), int(bool(update_prop)), int(bool(rician)),
                          ctypes.c_ulonglong(seed), ctypes.c_longlong(voxel_offset), samples.ctypes.data_as(_D),
-                         tau.ctypes.data_as(_D))
+                         tau.ctypes.data_as(_D), _seconds_ptr(seconds, nvox))
         return p, samples, (tau if rician else None)
 
     def predicted(self, samples, cfp, fixp=None):

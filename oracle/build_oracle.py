@@ -31,6 +31,7 @@ FLAGS = ["-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fopenmp", "-fPIC", "-
 UNIT = """// generated
 #include "host_abi.h"
 #include "philox_ref.h"
+#include <omp.h>
 #include <stdint.h>
 #undef MODEL_H_INCLUDED
 #undef NPARAMS

@@ -13,7 +13,8 @@ ACQ = {
 }
 
 MODEL_ACQ = {"Ball_1_Stick": "M64", "Ball_2_Sticks": "M280", "Ball_3_Sticks": "M280", "DiffusionTensor": "M64",
-             "toymodel": "M20", "NODDI_Watson": "M99", "NODDI_Bingham": "M99", "NODDI_Bingham_2Fans": "M99"}
+             "toymodel": "M20", "NODDI_Watson": "M99", "NODDI_Watson_refnum": "M99", "NODDI_Bingham": "M99",
+             "NODDI_Bingham_2Fans": "M99"}
 
 
 def truth(model, n, seed):
@@ -27,7 +28,7 @@ def truth(model, n, seed):
         return synth.truth_dti(n, seed)
     if model == "toymodel":
         return synth.truth_toymodel(n, seed)
-    if model == "NODDI_Watson":
+    if model in ("NODDI_Watson", "NODDI_Watson_refnum"):
         return synth.truth_noddi_watson(n, seed)
     if model == "NODDI_Bingham":
         return synth.truth_noddi_bingham(n, seed)

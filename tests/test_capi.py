@@ -136,3 +136,35 @@ def test_tuning_table_nearest_measurement_count(tmp_path):
     t.write_text("NODDI_Bingham 99 levmar 64 3.5 8\n")
     with pytest.raises(capi.CdmError, match="register budget"):
         m.load_tuning(str(t))
+
+
+def test_packed_triangle_index_host_build(tmp_path):
+    """The packed lower-triangle index the 64-lane LM sums hand out (entry e
+    to lane e % 64) maps every e back to its (i, j) for 1..32 parameters,
+    NODDI_Bingham_2Fans' 14 (105 entries, two rounds of the wave) included:
+    the engine header compiled for the host only and run here."""
+    import subprocess
+    csrc = os.path.join(os.path.dirname(capi.LIB_PATH), "..", "csrc")
+    src = tmp_path / "tri.hip"
+    checks = "\n".join("  ok &= check<%d>();" % n for n in range(1, 33))
+    src.write_text('#include "engine/packed.h"\n#include <cstdio>\n'
+                   "template <int NP> bool check() {\n"
+                   "  int e = 0;\n"
+                   "  for (int i = 0; i < NP; ++i)\n"
+                   "    for (int j = 0; j <= i; ++j, ++e) {\n"
+                   "      int a = -1, b = -1;\n"
+                   "      cdm::tri_ij<NP>(e, a, b);\n"
+                   "      if (a != i || b != j) { std::printf(\"NP=%d e=%d -> (%d,%d), want (%d,%d)\\n\", NP, e, a, b, i, j); return false; }\n"
+                   "    }\n"
+                   "  return e == cdm::tri_size(NP) && cdm::tri_ij_exact<NP>();\n"
+                   "}\n"
+                   "int main() {\n  bool ok = true;\n" + checks + "\n  std::printf(ok ? \"ok\\n\" : \"bad\\n\");\n"
+                   "  return ok ? 0 : 1;\n}\n")
+    exe = tmp_path / "tri"
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not installed")
+    subprocess.run([hipcc, "--offload-host-only", "-std=c++17", "-I", csrc, str(src), "-o", str(exe)], check=True,
+                   capture_output=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout

@@ -24,9 +24,9 @@ from tests import helpers as H
 pytestmark = pytest.mark.gpu
 
 SIMPLE = ["Ball_1_Stick", "Ball_2_Sticks", "Ball_3_Sticks", "DiffusionTensor", "toymodel"]
-NODDI = ["NODDI_Watson", "NODDI_Bingham", "NODDI_Bingham_2Fans"]
+NODDI = ["NODDI_Watson", "NODDI_Watson_refnum", "NODDI_Bingham", "NODDI_Bingham_2Fans"]
 # FP32 forward vs FP64: relative to the voxel's largest signal
-SIGNAL_TOL = {"NODDI_Watson": 1e-4, "NODDI_Bingham": 1e-4, "NODDI_Bingham_2Fans": 1e-4}
+SIGNAL_TOL = {"NODDI_Watson": 1e-4, "NODDI_Watson_refnum": 1e-4, "NODDI_Bingham": 1e-4, "NODDI_Bingham_2Fans": 1e-4}
 
 
 def _models():
@@ -90,6 +90,8 @@ GRIDS = {
     # Pipeline_NODDI_Bingham.sh:204-207), orientations held at the truth
     "NODDI_Watson": "search[0]=(0.0,0.1,0.2,0.3,0.4,0.5,0.6,0.7,0.8,0.9,1.0)\n"
                     "search[1]=(0.0,0.1,0.2,0.3,0.4,0.5,0.6,0.7,0.8,0.9,1.0)\nsearch[2]=(1,2,3,4,5,6,7,8)\n",
+    "NODDI_Watson_refnum": "search[0]=(0.0,0.1,0.2,0.3,0.4,0.5,0.6,0.7,0.8,0.9,1.0)\n"
+                           "search[1]=(0.0,0.1,0.2,0.3,0.4,0.5,0.6,0.7,0.8,0.9,1.0)\nsearch[2]=(1,2,3,4,5,6,7,8)\n",
     "NODDI_Bingham": "search[0]=(0.0,0.1,0.3)\nsearch[1]=(0.3,0.6,0.8,0.95)\nsearch[2]=(0.5,2,5,10,20,40)\n"
                      "search[3]=(0.2,1.0,3.0,9,20)\n",
 }
@@ -124,18 +126,63 @@ def test_gridsearch_argmin_exact(torch_cuda, model, lanes):
     np.testing.assert_allclose(H.host(dp), p_ref, rtol=1e-6, atol=1e-7)
 
 
+def test_watson_refnum_gridsearch_matches_reference_plugin(torch_cuda):
+    """NODDI_Watson_refnum (the reference's numerical forms) on the GPU picks
+    the same GridSearch combination as the oracle engine over the
+    reference's own NODDI_Watson plug-in compiled from its sources
+    (oracle/_ref), on the reference pipeline's grid with the orientation at
+    the truth; and counts how many argmins NODDI_Watson's corrected
+    numerics move."""
+    from cudimot_amd import capi
+    if not bind.ref_available():
+        pytest.skip("oracle/_ref (the reference's plug-ins compiled from source) is not built here")
+    n = 600
+    ref = bind.OracleModel("NODDI_Watson", ref=True)
+    cfp, t, fx, noisy = H.volume(bind.OracleModel("NODDI_Watson"), "NODDI_Watson", n, seed=23)
+    p0 = _clip_noddi("NODDI_Watson", t.copy())
+    kept = {}
+    for model in ("NODDI_Watson_refnum", "NODDI_Watson"):
+        m = capi.Model(model)
+        gp, grid = m.load_grid(GRIDS["NODDI_Watson"], is_text=True)
+        k = torch_cuda.full((n,), -7, dtype=torch_cuda.int32, device="cuda")
+        m.gridsearch(H.dev(noisy), H.dev(p0), H.dev(cfp), H.dev(fx), kept=k)
+        capi.sync()
+        kept[model] = k.cpu().numpy()
+    spec = _spec_arrays(capi.Model("NODDI_Watson_refnum").spec())
+    _, kept_ref = ref.gridsearch(spec, noisy.astype(np.float32).astype(np.float64), p0.astype(np.float32).astype(
+        np.float64), cfp.astype(np.float32).astype(np.float64), fx, np.asarray(gp), np.asarray(grid))
+    moved = np.mean(kept["NODDI_Watson"] != kept_ref)
+    print("refnum vs reference plug-in: %d mismatches; NODDI_Watson moves %.2f%% of the argmins" %
+          (np.sum(kept["NODDI_Watson_refnum"] != kept_ref), 100 * moved))
+    np.testing.assert_array_equal(kept["NODDI_Watson_refnum"], kept_ref)
+
+
 def _clip_noddi(model, p):
     """Ground truths inside the reference's bounds/constraints (beta <= 32,
     beta < kappa - 0.1)."""
+    if model == "NODDI_Bingham_2Fans":
+        from cudimot_amd import synth
+        return synth.clip_to_bounds(model, p)
     if model == "NODDI_Bingham":
         p[:, 2] = np.clip(p[:, 2], 0.2, 63.0)
         p[:, 3] = np.clip(np.minimum(p[:, 3], p[:, 2] - 0.2), 0.02, 31.0)
-    if model == "NODDI_Watson":
+    if model.startswith("NODDI_Watson"):
         p[:, 2] = np.clip(p[:, 2], 0.01, 63.0)
     return p
 
 
 def _near_truth(model, t, rng):
+    if model == "NODDI_Bingham_2Fans":
+        p = _clip_noddi(model, t.copy())
+        n = len(p)
+        p[:, 0] = np.clip(p[:, 0] + 0.05 * rng.normal(size=n), 0.01, 0.9)
+        p[:, 1] = np.clip(p[:, 1] + 0.05 * rng.normal(size=n), 0.05, 0.45)
+        for f, k, b, a in ((2, 3, 4, 5), (8, 9, 10, 11)):
+            p[:, f] = np.clip(p[:, f] + 0.05 * rng.normal(size=n), 0.05, 0.95)
+            p[:, k] = np.clip(p[:, k] * (1 + 0.1 * rng.normal(size=n)), 0.3, 60)
+            p[:, b] = np.clip(np.minimum(p[:, b] * (1 + 0.1 * rng.normal(size=n)), p[:, k] - 0.6), 0.02, 31)
+            p[:, a:a + 3] += 0.05 * rng.normal(size=(n, 3))
+        return p
     if model.startswith("NODDI"):
         p = _clip_noddi(model, t.copy())
         n = len(p)
@@ -200,7 +247,8 @@ def _run_levmar(torch_cuda, model, lanes, start, n=None):
     from cudimot_amd import capi
     m = capi.Model(model)
     m.set_group(capi.LEVMAR, lanes)
-    n = n or {"Ball_2_Sticks": 256, "Ball_3_Sticks": 256, "NODDI_Watson": 256, "NODDI_Bingham": 1024}.get(model, 1024)
+    n = n or {"Ball_2_Sticks": 256, "Ball_3_Sticks": 256, "NODDI_Watson": 256, "NODDI_Bingham": 1024,
+              "NODDI_Bingham_2Fans": 128}.get(model, 1024)
     o, cfp, fx, noisy, p0, noisy32, cfp32, p_ref, c_ref, spread = _oracle_levmar(model, start, n)
     dp = H.dev(p0)
     cost = torch_cuda.zeros((n,), dtype=torch_cuda.float32, device="cuda")
@@ -255,22 +303,44 @@ def test_levmar_parity(torch_cuda, model, lanes):
 
 @pytest.mark.parametrize("model", ["Ball_2_Sticks", "Ball_3_Sticks"])
 @pytest.mark.parametrize("start", ["init", "near"])
-def test_levmar_multistick_unbiased(torch_cuda, model, start):
+@pytest.mark.parametrize("lanes", [16, 64])
+def test_levmar_multistick_unbiased(torch_cuda, model, start, lanes):
     """Multi-stick fits are multi-modal (and Ball_2_Sticks' ph2 derivative is
     the reference's th2 one), so FP32 vs FP64 rounding can pick another
     nearby minimum: the GPU must not land in worse minima more often than
-    the FP64 oracle does, and most voxels must agree."""
-    p, p_ref, c_at, c_ref, c, _ = _run_levmar(torch_cuda, model, 16, start)
+    the FP64 oracle does, and most voxels must agree.  At 64 lanes the
+    normal equations are summed through LDS in two rounds of the wave for
+    Ball_3_Sticks (77 sums)."""
+    p, p_ref, c_at, c_ref, c, _ = _run_levmar(torch_cuda, model, lanes, start)
     lr = np.log(c_at / c_ref)
     worse, better = np.mean(lr > 1e-3), np.mean(lr < -1e-3)
-    print(model, start, "GPU worse", worse, "better", better)
+    print(model, start, lanes, "GPU worse", worse, "better", better)
     assert abs(np.median(lr)) < 1e-3, np.median(lr)
     assert worse <= better + 0.05, (worse, better)
     if start == "near":
         assert np.mean(np.abs(lr) <= 1e-3) >= 0.8
 
 
-@pytest.mark.parametrize("model", ["Ball_1_Stick", "Ball_2_Sticks", "toymodel", "NODDI_Watson", "NODDI_Bingham"])
+@pytest.mark.parametrize("lanes", [16, 64])
+def test_levmar_2fans_unbiased(torch_cuda, lanes):
+    """NODDI_Bingham_2Fans from starts near the truth (the tuned launch is
+    64 lanes, whose LDS normal sums take two rounds of the wave for its 119
+    entries; 16 lanes keep per-lane accumulators).  The fit is chaotic (two
+    fans, flat fanning angles), so the GPU must not end in worse minima more
+    often than better ones and most voxels must agree."""
+    p, p_ref, c_at, c_ref, c, _ = _run_levmar(torch_cuda, "NODDI_Bingham_2Fans", lanes, "near")
+    lr = np.log(c_at / c_ref)
+    worse, better, same = np.mean(lr > 1e-3), np.mean(lr < -1e-3), np.mean(np.abs(lr) <= 1e-3)
+    print("2Fans", lanes, "GPU worse", worse, "better", better, "same", same, "median", np.median(lr))
+    assert np.all(np.isfinite(p))
+    assert abs(np.median(lr)) < 1e-3, np.median(lr)
+    assert worse <= better + 0.05, (worse, better)
+    assert same >= 0.5, same
+    assert np.median(np.abs(c - c_at) / np.maximum(c_at, 1e-30)) < 1e-4
+
+
+@pytest.mark.parametrize("model", ["Ball_1_Stick", "Ball_2_Sticks", "Ball_3_Sticks", "DiffusionTensor", "toymodel",
+                                   "NODDI_Watson", "NODDI_Bingham", "NODDI_Bingham_2Fans"])
 @pytest.mark.parametrize("rician", [False, True])
 def test_mcmc_draw_for_draw(torch_cuda, model, rician):
     from cudimot_amd import capi
@@ -326,7 +396,49 @@ def test_mcmc_posterior_moments(torch_cuda, model):
     assert 0.8 <= np.median(sd_ratio) <= 1.25, np.median(sd_ratio, 0)
 
 
-@pytest.mark.parametrize("model", SIMPLE)
+LONG_IDX = {"NODDI_Bingham": [0, 1, 2, 3], "NODDI_Watson": [0, 1, 2], "Ball_2_Sticks": [0, 1, 2, 5]}
+
+
+@pytest.mark.parametrize("model", sorted(LONG_IDX))
+def test_mcmc_long_chain_reference_defaults(torch_cuda, model):
+    """Chains at the reference's defaults (1000 burn-in jumps, 1250 jumps, a
+    sample every 25; MCMC.cu:327-683) from the truth, GPU vs oracle with the
+    same seed: the two chains follow the same draws until FP32 vs FP64
+    rounding flips an acceptance, then sample the same posterior
+    independently.  Means of the scalar parameters (fractions, kappa,
+    beta, S0, d) within 0.35 posterior SD (median over voxels), SD ratio
+    within 0.8-1.25 (median)."""
+    from cudimot_amd import capi
+    o = bind.OracleModel(model)
+    m = capi.Model(model)
+    spec = m.spec()
+    n = 256
+    cfp, t, fx, noisy = H.volume(o, model, n, seed=57)
+    start = _clip_noddi(model, t.copy())
+    noisy32 = noisy.astype(np.float32).astype(np.float64)
+    cfp32 = cfp.astype(np.float32).astype(np.float64)
+    start32 = start.astype(np.float32).astype(np.float64)
+    nb, nj, se = 1000, 1250, 25
+    _, s_ref, _ = o.mcmc(_spec_arrays(spec), noisy32, start32, cfp32, fx, nburn=nb, njumps=nj, sampleevery=se,
+                         seed=2024)
+    samples = torch_cuda.zeros((n, m.nparams, nj // se), dtype=torch_cuda.float32, device="cuda")
+    m.mcmc(H.dev(noisy), H.dev(start), H.dev(cfp), H.dev(fx), samples, None, nburn=nb, njumps=nj, sampleevery=se,
+           seed=2024)
+    capi.sync()
+    s = H.host(samples)
+    idx = LONG_IDX[model]
+    mu_g, mu_o = s[:, idx].mean(2), s_ref[:, idx].mean(2)
+    sd_o = s_ref[:, idx].std(2) + 1e-30
+    z = np.abs(mu_g - mu_o) / sd_o
+    same = np.mean(np.all(np.abs(s - s_ref) <= 1e-4 * np.maximum(np.abs(s_ref), 1e-6), axis=(1, 2)))
+    sd_ratio = s[:, idx].std(2) / sd_o
+    print(model, "identical chains", same, "median z", np.median(z, 0), "median sd ratio", np.median(sd_ratio, 0))
+    assert np.all(np.isfinite(s))
+    assert np.all(np.median(z, 0) <= 0.35), np.median(z, 0)
+    assert np.all((np.median(sd_ratio, 0) >= 0.8) & (np.median(sd_ratio, 0) <= 1.25)), np.median(sd_ratio, 0)
+
+
+@pytest.mark.parametrize("model", SIMPLE + NODDI)
 @pytest.mark.parametrize("rician", [False, True])
 def test_bic_aic_parity(torch_cuda, model, rician):
     from cudimot_amd import capi
