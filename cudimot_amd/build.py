@@ -182,11 +182,21 @@ def batch_unit_source(name, model_dirs):
     return "\n".join(out)
 
 
-def build(models=None, extra_dirs=(), force=False, jobs=None, batches=None):
-    os.makedirs(BUILD, exist_ok=True)
-    os.makedirs(LIBDIR, exist_ok=True)
+def build(models=None, extra_dirs=(), force=False, jobs=None, batches=None, out_dir=None):
+    """Compile the engine library.  models: shipped plug-ins (None: all, []:
+    none); extra_dirs: further plug-in directories in the reference's
+    format, each compiled unmodified under its directory's name.  out_dir:
+    build tree and library under out_dir (out_dir/_build,
+    out_dir/lib/libcudimot.so) instead of the package's own, with no host
+    driver or FLOP-model library; a scratch build never replaces the
+    package's library."""
+    build_dir, lib_dir = (BUILD, LIBDIR) if out_dir is None else (os.path.join(out_dir, "_build"),
+                                                                    os.path.join(out_dir, "lib"))
+    lib_path = os.path.join(lib_dir, "libcudimot.so")
+    os.makedirs(build_dir, exist_ok=True)
+    os.makedirs(lib_dir, exist_ok=True)
     dirs = {}
-    for m in (models or shipped_models()):
+    for m in (shipped_models() if models is None else models):
         dirs[m] = os.path.join(MODELS, m)
     for d in extra_dirs:
         d = os.path.abspath(d)
@@ -195,16 +205,16 @@ def build(models=None, extra_dirs=(), force=False, jobs=None, batches=None):
     engine_deps = [os.path.join(CSRC, "engine"), os.path.join(CSRC, "abi"), os.path.join(CSRC, "host", "model_ops.h")]
     jobs_list = []
     for name, d in sorted(dirs.items()):
-        src = os.path.join(BUILD, "unit_%s.hip" % name)
+        src = os.path.join(build_dir, "unit_%s.hip" % name)
         text = template.replace("@NAME@", name).replace("@DIR@", d)
         if not os.path.exists(src) or open(src).read() != text:
             with open(src, "w") as f:
                 f.write(text)
-        jobs_list.append((src, os.path.join(BUILD, "unit_%s.o" % name), engine_deps + sorted(_plugin_deps(d))))
+        jobs_list.append((src, os.path.join(build_dir, "unit_%s.o" % name), engine_deps + sorted(_plugin_deps(d))))
     batches = DEFAULT_BATCHES if batches is None else batches
     batches = {b: ms for b, ms in batches.items() if all(m in dirs for m in ms)}
     for bname, ms in sorted(batches.items()):
-        src = os.path.join(BUILD, "batch_%s.hip" % bname)
+        src = os.path.join(build_dir, "batch_%s.hip" % bname)
         text = batch_unit_source(bname, [(m, dirs[m]) for m in ms])
         if not os.path.exists(src) or open(src).read() != text:
             with open(src, "w") as f:
@@ -212,8 +222,8 @@ def build(models=None, extra_dirs=(), force=False, jobs=None, batches=None):
         bdeps = set()
         for m in ms:
             _plugin_deps(dirs[m], bdeps)
-        jobs_list.append((src, os.path.join(BUILD, "batch_%s.o" % bname), engine_deps + sorted(bdeps)))
-    reg = os.path.join(BUILD, "registry.cpp")
+        jobs_list.append((src, os.path.join(build_dir, "batch_%s.o" % bname), engine_deps + sorted(bdeps)))
+    reg = os.path.join(build_dir, "registry.cpp")
     lines = ['#include "host/model_ops.h"', "#include <cstring>"]
     for name in sorted(dirs):
         lines.append("namespace cdm_model_%s { extern const cdm::ModelOps ops; }" % name)
@@ -236,14 +246,14 @@ def build(models=None, extra_dirs=(), force=False, jobs=None, batches=None):
         with open(reg, "w") as f:
             f.write(text)
     host_deps = [os.path.join(CSRC, "host"), os.path.join(PKG, "..", "include", "cudimot_amd.h")]
-    jobs_list.append((reg, os.path.join(BUILD, "registry.o"), host_deps))
+    jobs_list.append((reg, os.path.join(build_dir, "registry.o"), host_deps))
     for f in ("capi.cpp", "spec.cpp"):
-        jobs_list.append((os.path.join(CSRC, "host", f), os.path.join(BUILD, f[:-4] + ".o"), host_deps))
-    jobs_list.append((os.path.join(CSRC, "mlp", "mlp.hip"), os.path.join(BUILD, "mlp.o"),
+        jobs_list.append((os.path.join(CSRC, "host", f), os.path.join(build_dir, f[:-4] + ".o"), host_deps))
+    jobs_list.append((os.path.join(CSRC, "mlp", "mlp.hip"), os.path.join(build_dir, "mlp.o"),
                       host_deps + [os.path.join(CSRC, "mlp"), os.path.join(CSRC, "engine", "philox.h")]))
-    jobs_list.append((os.path.join(CSRC, "init", "orientation.hip"), os.path.join(BUILD, "orientation.o"),
+    jobs_list.append((os.path.join(CSRC, "init", "orientation.hip"), os.path.join(build_dir, "orientation.o"),
                       host_deps + [os.path.join(CSRC, "init")]))
-    jobs_list.append((os.path.join(CSRC, "finish", "derived.hip"), os.path.join(BUILD, "derived.o"),
+    jobs_list.append((os.path.join(CSRC, "finish", "derived.hip"), os.path.join(build_dir, "derived.o"),
                       host_deps + [os.path.join(CSRC, "finish")]))
     if force:
         for _, o, _ in jobs_list:
@@ -252,12 +262,13 @@ def build(models=None, extra_dirs=(), force=False, jobs=None, batches=None):
     n = jobs or min(8, os.cpu_count() or 4)
     with ThreadPoolExecutor(max_workers=n) as ex:
         objs = list(ex.map(lambda j: _compile(*j), jobs_list))
-    tmp = LIB + ".tmp"
+    tmp = lib_path + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-ldl"])
-    os.replace(tmp, LIB)
-    build_cli()
-    build_opcount(models)
-    return LIB
+    os.replace(tmp, lib_path)
+    if out_dir is None:
+        build_cli()
+        build_opcount(models)
+    return lib_path
 
 
 BINDIR = os.path.join(PKG, "bin")
@@ -332,16 +343,20 @@ def build_opcount(models=None):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
-    ap.add_argument("--models", default="", help="comma-separated shipped plug-ins (default: all)")
-    ap.add_argument("--extra-model-dir", action="append", default=[], help="a plug-in directory to add")
+    ap.add_argument("--models", default="", help="comma-separated shipped plug-ins (default: all; 'none': none)")
+    ap.add_argument("--extra-model-dir", action="append", default=[],
+                    help="a plug-in directory in the reference's format to add, compiled unmodified")
+    ap.add_argument("--out", default=None,
+                    help="build into this directory (OUT/lib/libcudimot.so) instead of the package's library")
     ap.add_argument("--batch", action="append", default=[],
                     help="a batch unit NAME=ModelA,ModelB,... (default: %s)" % DEFAULT_BATCHES)
     ap.add_argument("--force", action="store_true")
     a = ap.parse_args(argv)
-    models = [m for m in a.models.split(",") if m] or None
+    models = [] if a.models == "none" else ([m for m in a.models.split(",") if m] or None)
     batches = dict((b.split("=", 1)[0], b.split("=", 1)[1].split(",")) for b in a.batch) if a.batch else None
-    print(build(models, a.extra_model_dir, a.force, batches=batches))
-    print(OPC_LIB)
+    print(build(models, a.extra_model_dir, a.force, batches=batches, out_dir=a.out))
+    if a.out is None:
+        print(OPC_LIB)
 
 
 if __name__ == "__main__":

@@ -10,6 +10,10 @@ Builds two CPU libraries with g++:
   present: the same engine instantiated with the reference's own plug-in
   headers, compiled from where they lie under /root/reference/mymodels.  Its
   outputs pin this project's plug-ins and feed the golden fixtures.
+* ``oracle/_ref/dropin/lib/libcudimot.so`` -- only when the reference
+  checkout is present: the GPU engine library built by cudimot_amd.build
+  over reference plug-in directories, unmodified (the plug-in drop-in the
+  tests run against the oracle over the same headers).
 
 Run: ``python oracle/build_oracle.py``.  Outputs go only to oracle/_build and
 oracle/_ref (both git-ignored).
@@ -119,8 +123,32 @@ def build_ref():
     return _build_lib(os.path.join(HERE, "_ref"), "libref_models.so", units)
 
 
+# Reference plug-ins compiled unmodified through the engine's own build (the
+# plug-in drop-in): a scratch engine library holding only these models.
+DROPIN_SET = ["Ball_1_Stick", "NODDI_Watson"]
+DROPIN_DIR = os.path.join(HERE, "_ref", "dropin")
+DROPIN_LIB = os.path.join(DROPIN_DIR, "lib", "libcudimot.so")
+
+
+def build_dropin():
+    """The engine library (cudimot_amd.build, gfx950) over the reference's own
+    plug-in directories, unmodified, into oracle/_ref/dropin; only when the
+    reference checkout is present."""
+    dirs = [os.path.join(REF_MODELS, m) for m in DROPIN_SET if os.path.isdir(os.path.join(REF_MODELS, m))]
+    if not dirs:
+        return None
+    sys.path.insert(0, REPO)
+    try:
+        from cudimot_amd import build as engine_build
+    finally:
+        sys.path.pop(0)
+    return engine_build.build(models=[], extra_dirs=dirs, batches={}, out_dir=DROPIN_DIR)
+
+
 if __name__ == "__main__":
     print(build_oracle())
     r = build_ref()
     print(r if r else "reference checkout absent: oracle/_ref not built")
+    d = build_dropin()
+    print(d if d else "reference checkout absent: oracle/_ref/dropin not built")
     sys.exit(0)
