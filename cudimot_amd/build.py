@@ -93,7 +93,8 @@ DEFAULT_BATCHES = {"tensor_bingham2_sticks3": ["DiffusionTensor", "NODDI_Bingham
 PLUGIN_MACROS = ["MODEL_H_INCLUDED", "NPARAMS", "NCFP", "NFIXP", "CDM_LM_JACOBIAN_DOUBLE", "CDM_VOXEL_CACHE",
                  "CDM_VOXEL_CACHE_DEPENDS", "CDM_MEAS_CACHE", "CDM_NUMERICAL_MASK", "CDM_MEAS_VOXEL_CACHE",
                  "CDM_MEAS_VOXEL_CACHE_DEPENDS", "CDM_VOXEL_CACHE_JOBS", "CDM_VOXEL_CACHE_JOBS_DEPENDS",
-                 "CDM_MEAS_VOXEL_CACHE_FILL", "CDM_SIGNAL_PARTIALS", "CDM_WATSON_REFERENCE_FORMS"]
+                 "CDM_MEAS_VOXEL_CACHE_FILL", "CDM_SIGNAL_PARTIALS", "CDM_WATSON_REFERENCE_FORMS",
+                 "CDM_LM_JACOBIAN_COLUMNS"]
 
 
 def batch_unit_source(name, model_dirs):
@@ -142,22 +143,27 @@ def batch_unit_source(name, model_dirs):
             "                   size_t* lds_bytes) {",
             "  DevItem* out = static_cast<DevItem*>(host_out);", "  int tiles = 0;", "  size_t lds = 0;",
             "  for (int i = 0; i < nitems; ++i) {", "    const cdm::BatchLaunchItem& b = in[i];",
-            "    const int li = b.L == 1 ? 0 : b.L == 4 ? 1 : b.L == 16 ? 2 : b.L == 64 ? 3 : -1;",
-            "    if (li < 0 || b.model < 0 || b.model >= %d || b.g.nvox <= 0) return -1;" % len(model_dirs),
-            "    DevItem& d = out[i];", "    std::memset(&d, 0, sizeof(d));", "    d.kase = 4 * b.model + li;",
-            "    d.first_tile = tiles;", "    const int per = 256 / b.L;", "    d.ntiles = (b.g.nvox + per - 1) / per;",
-            "    tiles += d.ntiles;", "    size_t need = 0;", "    switch (b.model) {"]
+            "    if (b.model < 0 || b.model >= %d || b.g.nvox <= 0) return -1;" % len(model_dirs),
+            "    DevItem& d = out[i];", "    std::memset(&d, 0, sizeof(d));",
+            "    int L = b.L;", "    size_t need = 0;", "    switch (b.model) {"]
     for k in range(len(model_dirs)):
         out += ["      case %d: {" % k,
                 "        if (b.g.cfp_ts > %s::cdm_cfp_max || b.g.fixp_ts > %s::cdm_fixp_max) return -1;" % (ns(k), ns(k)),
-                "        need = mode == cdm::kBatchMcmc ? %s::op_lds_mcmc(b.g, b.L) : %s::op_lds_lm(b.g, b.L);" % (ns(k), ns(k)),
+                "        // LM groups of 64 lanes past the LDS fall back to 16, as the per-model launch",
+                "        if (mode == cdm::kBatchLevMar) L = %s::op_lm_lanes(b.g, L);" % ns(k),
+                "        need = mode == cdm::kBatchMcmc ? %s::op_lds_mcmc(b.g, L) : %s::op_lds_lm(b.g, L);" % (ns(k), ns(k)),
                 "        if (need > %s::cdm_lds_max) return -1;" % ns(k),
                 "        if (mode == cdm::kBatchLevMar) {",
                 "          const auto a = %s::op_lm_args(b.g, b.params, b.max_iters, b.c, b.iters, b.cost);" % ns(k),
                 "          std::memcpy(d.args, &a, sizeof(a));", "        } else {",
-                "          const auto a = %s::op_mcmc_args(b.g, b.mc, b.c);" % ns(k),
+                "          const auto a = %s::op_mcmc_args(b.g, L, b.mc, b.c);" % ns(k),
                 "          std::memcpy(d.args, &a, sizeof(a));", "        }", "      } break;"]
-    out += ["      default: return -1;", "    }", "    lds = std::max(lds, need);", "  }",
+    out += ["      default: return -1;", "    }",
+            "    const int li = L == 1 ? 0 : L == 4 ? 1 : L == 16 ? 2 : L == 64 ? 3 : -1;",
+            "    if (li < 0) return -1;",
+            "    d.kase = 4 * b.model + li;", "    d.first_tile = tiles;", "    const int per = 256 / L;",
+            "    d.ntiles = (b.g.nvox + per - 1) / per;", "    tiles += d.ntiles;",
+            "    lds = std::max(lds, need);", "  }",
             "  *total_tiles = tiles;", "  *lds_bytes = lds;", "  return 0;", "}", "",
             "static hipError_t grid(int mode, int total_tiles, size_t lds, int* out) {",
             "  auto kern = mode == cdm::kBatchLevMar ? batch_kernel<0> : batch_kernel<1>;",
@@ -247,8 +253,8 @@ def build(models=None, extra_dirs=(), force=False, jobs=None, batches=None, out_
             f.write(text)
     host_deps = [os.path.join(CSRC, "host"), os.path.join(PKG, "..", "include", "cudimot_amd.h")]
     jobs_list.append((reg, os.path.join(build_dir, "registry.o"), host_deps))
-    for f in ("capi.cpp", "spec.cpp"):
-        jobs_list.append((os.path.join(CSRC, "host", f), os.path.join(build_dir, f[:-4] + ".o"), host_deps))
+    for f in ("capi.cpp", "spec.cpp", "fill.hip"):
+        jobs_list.append((os.path.join(CSRC, "host", f), os.path.join(build_dir, f.split(".")[0] + ".o"), host_deps))
     jobs_list.append((os.path.join(CSRC, "mlp", "mlp.hip"), os.path.join(build_dir, "mlp.o"),
                       host_deps + [os.path.join(CSRC, "mlp"), os.path.join(CSRC, "engine", "philox.h")]))
     jobs_list.append((os.path.join(CSRC, "init", "orientation.hip"), os.path.join(build_dir, "orientation.o"),

@@ -54,6 +54,8 @@ def lib():
             "cdm_model_set_occupancy": (i, [vp, i, i]),
             "cdm_model_get_occupancy": (i, [vp, i, i]),
             "cdm_model_set_mcmc_blocks": (i, [vp, i, i]),
+            "cdm_model_graph_stats": (i, [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
+            "cdm_batch_graph_stats": (i, [ctypes.POINTER(i64), ctypes.POINTER(i64)]),
             "cdm_gridsearch": (i, [vp, i, i, vp, vp, vp, vp, vp, vp]),
             "cdm_levmar": (i, [vp, i, i, vp, vp, vp, vp, i, i, vp, vp, vp]),
             "cdm_mcmc": (i, [vp, i, i, vp, vp, vp, vp, i, i, i, i, i, u64, i64, vp, vp, vp]),
@@ -197,6 +199,12 @@ class Model:
     def set_mcmc_blocks(self, jumps_per_launch, use_graph=False):
         check(lib().cdm_model_set_mcmc_blocks(self.h, int(jumps_per_launch), 1 if use_graph else 0))
 
+    def graph_stats(self):
+        """(graphs recorded, graphs replayed from the handle's cache)."""
+        c, r = ctypes.c_longlong(), ctypes.c_longlong()
+        check(lib().cdm_model_graph_stats(self.h, ctypes.byref(c), ctypes.byref(r)))
+        return c.value, r.value
+
     # --- methods on device buffers (torch CUDA tensors, float32, contiguous)
     def gridsearch(self, meas, params, cfp, fixp=None, kept=None, stream=None):
         nvox, nmeas = meas.shape
@@ -234,6 +242,13 @@ class BatchItem(ctypes.Structure):
                 ("meas", ctypes.c_void_p), ("params", ctypes.c_void_p), ("cfp", ctypes.c_void_p),
                 ("fixp", ctypes.c_void_p), ("iters", ctypes.c_void_p), ("cost", ctypes.c_void_p),
                 ("samples", ctypes.c_void_p), ("tau_samples", ctypes.c_void_p), ("voxel_offset", ctypes.c_longlong)]
+
+
+def batch_graph_stats():
+    """(graphs recorded, graphs replayed) by cdm_batch_mcmc in this process."""
+    c, r = ctypes.c_longlong(), ctypes.c_longlong()
+    check(lib().cdm_batch_graph_stats(ctypes.byref(c), ctypes.byref(r)))
+    return c.value, r.value
 
 
 def batch_units():

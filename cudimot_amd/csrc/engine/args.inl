@@ -29,7 +29,20 @@ static cdm_geom op_geom(const cdm::Geom& g) {
   return k;
 }
 
+// MCMC keeps the per-measurement cache rows in LDS when the block's LDS then
+// stays within this bound (two blocks per CU), else in global memory.
+constexpr size_t cdm_mcmc_lds_rows_max = 80 * 1024;
+
+static size_t op_mcmc_rows_lds_bytes(const cdm::Geom& g, int L) {
+  if (cdm_mvc <= 0) return 0;
+  const size_t rows = (size_t)(cdm_block / L) * 2 * (size_t)cdm_mcmc_row_stride(g.nmeas) * sizeof(cdm_tf);
+  const size_t total = cdm_mcmc_rows_offset(g.nmeas, g.cfp_ts, L) + rows;
+  return total <= cdm_mcmc_lds_rows_max ? rows : 0;
+}
+
 static size_t op_lds_mcmc(const cdm::Geom& g, int L) {
+  const size_t rows = op_mcmc_rows_lds_bytes(g, L);
+  if (rows) return cdm_mcmc_rows_offset(g.nmeas, g.cfp_ts, L) + rows;
   return cdm_vslot_offset(g.nmeas, g.cfp_ts) + (size_t)(cdm_block / L) * sizeof(cdm_vslot);
 }
 
@@ -45,6 +58,13 @@ static size_t op_lds_lm(const cdm::Geom& g, int L) {
   const size_t extra = cdm_lm_scratch_doubles(L, g.nmeas);
   if (!extra) return op_lds(g);
   return cdm_lm_scratch_offset(g.nmeas, g.cfp_ts) + (size_t)(cdm_block / L) * extra * sizeof(CDM_F64);
+}
+
+// LM voxel-group size actually launched: 64-lane groups keep the rows of J
+// in LDS, and past the LDS of a CU they fall back to 16 lanes (per-lane
+// accumulators).
+static int op_lm_lanes(const cdm::Geom& g, int L) {
+  return (L == 64 && op_lds_lm(g, 64) > cdm_lds_max) ? 16 : L;
 }
 
 static dim3 op_grid(int nvox, int L) {
@@ -65,11 +85,14 @@ static cdm_lm_args op_lm_args(const cdm::Geom& g, float* params, int max_iters, 
   return a;
 }
 
-static cdm_mcmc_args op_mcmc_args(const cdm::Geom& g, const cdm::McmcLaunch& m, const cdm::Consts& c) {
+// L: the launch's voxel-group size (it sets the LDS layout of the rows).
+static cdm_mcmc_args op_mcmc_args(const cdm::Geom& g, int L, const cdm::McmcLaunch& m, const cdm::Consts& c) {
   cdm_mcmc_args a;
   a.g = op_geom(g);
   a.params = m.params;
   a.mvc = (cdm_tf*)m.mvc;
+  a.rows_lds = op_mcmc_rows_lds_bytes(g, L) > 0 ? 1 : 0;
+  a.row_stride = cdm_mcmc_row_stride(g.nmeas);
   a.state = (cdm_chain*)m.state;
   a.samples = m.samples;
   a.tau_samples = m.tau_samples;

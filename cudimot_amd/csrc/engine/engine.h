@@ -62,6 +62,10 @@ constexpr int cdm_ntri = cdm::tri_size(NPARAMS);
 //                              differences (the reference's NUMERICAL step
 //                              rule) with per-perturbation voxel caches;
 //                              Partial_Derivatives_Cached fills the others
+//   CDM_LM_JACOBIAN_COLUMNS    with CDM_NUMERICAL_MASK and the cache's jobs:
+//                              64-lane LM groups form the Jacobian column by
+//                              column (cdm_lm_jacobian_columns), holding two
+//                              probe caches at a time instead of all of them
 //   CDM_SIGNAL_PARTIALS        Predicted_Signal_Partials_Cached(npar, P, CFP,
 //                              FixP, vc, mc, d) returns the signal of
 //                              Predicted_Signal_Cached and fills the analytic
@@ -511,11 +515,20 @@ __device__ __forceinline__ void cdm_dtransform(const cdm_consts& c, const double
 // (step clamp(rel |p|, lo, hi), probes p + h/2 and p + h/2 - 2 (h/2)), using
 // voxel caches prepared once per perturbation; otherwise the plug-in's
 // Partial_Derivatives is called.
+#ifdef CDM_NUMERICAL_MASK
+// Whether the row-wise form keeps every probe's voxel cache for the whole
+// measurement loop; past this size (in doubles) it recomputes a probe's
+// cache where the probe is evaluated instead (64-lane groups of plug-ins
+// with CDM_LM_JACOBIAN_COLUMNS take the column-wise form, which keeps two).
+constexpr bool cdm_jac_keep_probes =
+    (1 + 2 * __builtin_popcount(CDM_NUMERICAL_MASK & cdm_vc_dep)) * cdm_vc <= 160;
+#endif
+
 template <class TJ>
 struct cdm_jacobian_setup {
   CDM_F64 vc0[cdm_vc];
 #ifdef CDM_NUMERICAL_MASK
-  CDM_F64 vcp[cdm_np][cdm_vc], vcm[cdm_np][cdm_vc];
+  CDM_F64 vcp[cdm_np][cdm_vc], vcm[cdm_np][cdm_vc];  // unused (removed) without cdm_jac_keep_probes
   TJ h[cdm_np];
 #endif
 #ifdef CDM_NUMERICAL_MASK
@@ -562,6 +575,10 @@ struct cdm_jacobian_setup {
       h[i] = hi;
     }
     constexpr int njobs = 1 + 2 * __builtin_popcount(CDM_NUMERICAL_MASK & cdm_vc_dep);
+    if constexpr (!cdm_jac_keep_probes) {
+      cdm_prepare_g<L>(p, fx, vc0, lane);
+      return;
+    }
 #ifdef CDM_VOXEL_CACHE_JOBS
     if constexpr (cdm_vc > 0 && njobs > 1 && L >= 2) {
       // every (cache, job) pair on its own lane: pair t = c * jobs + j
@@ -652,10 +669,21 @@ struct cdm_jacobian_setup {
       if (!((CDM_NUMERICAL_MASK >> i) & 1u)) continue;
       const bool dep = (cdm_vc_dep >> i) & 1u;
       const TJ keep = p[i];
-      p[i] += h[i] * (TJ)0.5;
-      const TJ up = cdm_signal(p, cm, fx, dep ? vcp[i] : vc0, mc);
-      p[i] -= (TJ)2 * (h[i] * (TJ)0.5);
-      const TJ dn = cdm_signal(p, cm, fx, dep ? vcm[i] : vc0, mc);
+      TJ up, dn;
+      if constexpr (cdm_jac_keep_probes) {
+        p[i] += h[i] * (TJ)0.5;
+        up = cdm_signal(p, cm, fx, dep ? vcp[i] : vc0, mc);
+        p[i] -= (TJ)2 * (h[i] * (TJ)0.5);
+        dn = cdm_signal(p, cm, fx, dep ? vcm[i] : vc0, mc);
+      } else {
+        CDM_F64 vq[cdm_vc];
+        p[i] += h[i] * (TJ)0.5;
+        if (dep) cdm_prepare(p, fx, vq);
+        up = cdm_signal(p, cm, fx, dep ? vq : vc0, mc);
+        p[i] -= (TJ)2 * (h[i] * (TJ)0.5);
+        if (dep) cdm_prepare(p, fx, vq);
+        dn = cdm_signal(p, cm, fx, dep ? vq : vc0, mc);
+      }
       p[i] = keep;
       d[i] = (up - dn) / h[i];
     }
@@ -685,9 +713,12 @@ __host__ __device__ __forceinline__ size_t cdm_lm_scratch_offset(int nmeas, int 
 }
 
 // Doubles of LDS scratch per LM voxel group: the [M][P+1] rows (J, r), then
-// the P(P+1)/2 + P sums; none below 64 lanes.
+// the P(P+1)/2 + P sums (J^T J packed, then J^T r; J^T J is kept there,
+// doubled and with the fixed parameters' rows and columns replaced, for the
+// damped solves of rejected steps), then the P(P+1)/2 entries of the
+// factorised damped matrix; none below 64 lanes.
 __host__ __device__ __forceinline__ size_t cdm_lm_scratch_doubles(int L, int nmeas) {
-  return L == 64 ? (size_t)nmeas * (cdm_np + 1) + cdm_ntri + cdm_np : 0;
+  return L == 64 ? (size_t)nmeas * (cdm_np + 1) + 2 * cdm_ntri + cdm_np : 0;
 }
 
 // LDS stores of one wave made visible to the wave's other lanes.
@@ -716,6 +747,169 @@ __device__ __forceinline__ void cdm_lds_normal_sums(const CDM_F64* rows, CDM_F64
     for (int m = 0; m < nmeas; ++m) acc += rows[m * (cdm_np + 1) + i] * rows[m * (cdm_np + 1) + j];
     sums[e] = acc;
   }
+}
+
+#if defined(CDM_LM_JACOBIAN_COLUMNS) && defined(CDM_NUMERICAL_MASK) && defined(CDM_VOXEL_CACHE) && \
+    defined(CDM_VOXEL_CACHE_JOBS)
+// Column-wise Jacobian of a 64-lane LM group into the LDS rows [M][P+1]
+// (J columns, then the residual): one pass for the signal, the residual and
+// the analytic entries, then one pass per central-difference parameter,
+// whose two probes' voxel caches the group computes once (their 1F1 jobs
+// spread over the lanes when the parameter moves them, a recombination of
+// the base jobs otherwise, the base cache when the cache does not depend on
+// it).  Only the base cache and the two probes' caches are live at a time;
+// the row-wise form keeps every probe's cache for the whole measurement
+// loop.  Each entry is the row-wise form's value: the same probes (the
+// reference's step rule and increment sequence), caches and quotient.
+constexpr bool cdm_lm_columns = true;
+template <int L>
+__device__ __forceinline__ void cdm_lm_jacobian_columns(const cdm_geom& g, const float* s_cfp, const CDM_F64* s_mc,
+                                                        const float* mv, cdm_jt* pj, cdm_jt* fxj, const cdm_jt* dtf,
+                                                        CDM_F64* rows, int lane) {
+  static_assert(L == 64, "column-wise Jacobian: one voxel per wave");
+  constexpr int np1 = cdm_np + 1;
+  cdm_cfp_row<cdm_jt> rowj;
+  CDM_F64 jr0[cdm_vc_jobs], vc0[cdm_vc];
+  cdm_cache_jobs_g<L>(pj, fxj, jr0, lane);
+  Voxel_Cache_Combine(cdm_np, pj, fxj, jr0, vc0);
+  // signal, residual and analytic entries
+  for (int m = lane; m < g.nmeas; m += L) {
+    cdm_jt* cm = rowj.at(s_cfp, m, g.cfp_ts);
+    cdm_jt d[cdm_np];
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) d[i] = (cdm_jt)0;
+#ifdef CDM_SIGNAL_PARTIALS
+    const cdm_jt sig = Predicted_Signal_Partials_Cached(cdm_np, pj, cm, fxj, vc0, cdm_mc_row(s_mc, m), d);
+#else
+    const cdm_jt sig = cdm_signal(pj, cm, fxj, vc0, cdm_mc_row(s_mc, m));
+    Partial_Derivatives_Cached(cdm_np, pj, cm, fxj, vc0, cdm_mc_row(s_mc, m), d);
+#endif
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i)
+      if (!((CDM_NUMERICAL_MASK >> i) & 1u)) rows[m * np1 + i] = (CDM_F64)(cdm_lm_acc)(d[i] * dtf[i]);
+    rows[m * np1 + cdm_np] = (CDM_F64)(cdm_lm_acc)(sig - (cdm_jt)mv[m]);
+  }
+  // one column per central-difference parameter (a loop, not unrolled: the
+  // parameter index selects by compares, the body is one measurement loop)
+#pragma unroll 1
+  for (int i = 0; i < cdm_np; ++i) {
+    if (!((CDM_NUMERICAL_MASK >> i) & 1u)) continue;
+    cdm_jt pi = (cdm_jt)0, di = (cdm_jt)0;
+#pragma unroll
+    for (int k = 0; k < cdm_np; ++k)
+      if (k == i) {
+        pi = pj[k];
+        di = dtf[k];
+      }
+    cdm_jt h = fabs_gpu((cdm_jt)::cdm_fd_step<cdm_jt>::rel * pi);
+    h = min_gpu(max_gpu(h, (cdm_jt)::cdm_fd_step<cdm_jt>::lo), (cdm_jt)::cdm_fd_step<cdm_jt>::hi);
+    const cdm_jt up = pi + h * (cdm_jt)0.5;
+    const cdm_jt dn = up - (cdm_jt)2 * (h * (cdm_jt)0.5);
+    cdm_jt pu[cdm_np], pd[cdm_np];
+#pragma unroll
+    for (int k = 0; k < cdm_np; ++k) {
+      pu[k] = k == i ? up : pj[k];
+      pd[k] = k == i ? dn : pj[k];
+    }
+    const bool dep = (cdm_vc_dep >> i) & 1u, jdep = (cdm_vcj_dep >> i) & 1u;
+    CDM_F64 vcu[cdm_vc], vcd[cdm_vc];
+    if (!dep) {
+#pragma unroll
+      for (int c = 0; c < cdm_vc; ++c) vcu[c] = vcd[c] = vc0[c];
+    } else if (jdep) {
+      // the two probes' jobs, pair t = probe * jobs + job, one per lane
+      constexpr int T = 2 * cdm_vc_jobs;
+      static_assert(T <= L, "one round of probe jobs");
+      const int t = lane < T ? lane : 0;
+      // element-wise select: a pointer select between two register arrays
+      // would place both in scratch memory
+      cdm_jt q[cdm_np];
+#pragma unroll
+      for (int k = 0; k < cdm_np; ++k) q[k] = t < cdm_vc_jobs ? pu[k] : pd[k];
+      const CDM_F64 mine = Voxel_Cache_Job(t % cdm_vc_jobs, cdm_np, q, fxj);
+      CDM_F64 ju[cdm_vc_jobs], jd[cdm_vc_jobs];
+#pragma unroll
+      for (int j = 0; j < cdm_vc_jobs; ++j) {
+        ju[j] = cdm::Group<L>::bcast(mine, j);
+        jd[j] = cdm::Group<L>::bcast(mine, cdm_vc_jobs + j);
+      }
+      Voxel_Cache_Combine(cdm_np, pu, fxj, ju, vcu);
+      Voxel_Cache_Combine(cdm_np, pd, fxj, jd, vcd);
+    } else {
+      Voxel_Cache_Combine(cdm_np, pu, fxj, jr0, vcu);
+      Voxel_Cache_Combine(cdm_np, pd, fxj, jr0, vcd);
+    }
+    for (int m = lane; m < g.nmeas; m += L) {
+      cdm_jt* cm = rowj.at(s_cfp, m, g.cfp_ts);
+      const cdm_jt su = cdm_signal(pu, cm, fxj, vcu, cdm_mc_row(s_mc, m));
+      const cdm_jt sd = cdm_signal(pd, cm, fxj, vcd, cdm_mc_row(s_mc, m));
+      rows[m * np1 + i] = (CDM_F64)(cdm_lm_acc)(((su - sd) / h) * di);
+    }
+  }
+}
+#else
+constexpr bool cdm_lm_columns = false;
+template <int L>
+__device__ __forceinline__ void cdm_lm_jacobian_columns(const cdm_geom&, const float*, const CDM_F64*, const float*,
+                                                        cdm_jt*, cdm_jt*, const cdm_jt*, CDM_F64*, int) {}
+#endif
+
+// The damped solve of a 64-lane LM group with the matrices in LDS: hs the
+// packed 2 J^T J (fixed parameters' rows and columns replaced), diag its
+// damped diagonal, fac the factor's storage.  The LDL^T factorisation runs
+// column by column, lane i computing row i's entry of each column (the
+// register form's arithmetic for every entry, in the same order); every
+// lane then substitutes for the whole step.  The per-voxel registers keep
+// only vectors of P values, not the P(P+1)/2-entry matrices.
+template <int L>
+__device__ __forceinline__ void cdm_lds_damped_solve(const CDM_F64* hs, CDM_F64* fac, const double* diag,
+                                                     const double* grad, double* step, double* dg, int lane) {
+  static_assert(L == 64 && cdm_np <= 64, "one row per lane");
+  for (int e = lane; e < cdm_ntri; e += L) {
+    int i = 0, j = 0;
+    cdm::tri_ij<cdm_np>(e, i, j);
+    double di = 0.0;
+#pragma unroll
+    for (int k = 0; k < cdm_np; ++k)
+      if (k == i) di = diag[k];
+    fac[e] = (i == j) ? di : (double)hs[e];
+  }
+  cdm_wave_lds_sync();
+#pragma unroll
+  for (int j = 0; j < cdm_np; ++j) {
+    double dj = fac[j * (j + 1) / 2 + j];
+#pragma unroll
+    for (int q = 0; q < j; ++q) dj -= fac[j * (j + 1) / 2 + q] * fac[j * (j + 1) / 2 + q] * dg[q];
+    dg[j] = dj;
+    const double inv = 1.0 / dj;
+    const int i = lane;
+    if (i > j && i < cdm_np) {
+      const int ri = i * (i + 1) / 2;
+      double x = fac[ri + j];
+#pragma unroll
+      for (int q = 0; q < j; ++q) x -= fac[ri + q] * fac[j * (j + 1) / 2 + q] * dg[q];
+      fac[ri + j] = x * inv;
+    }
+    cdm_wave_lds_sync();
+  }
+#pragma unroll
+  for (int i = 0; i < cdm_np; ++i) {  // L y = g
+    double x = grad[i];
+#pragma unroll
+    for (int q = 0; q < i; ++q) x -= fac[i * (i + 1) / 2 + q] * step[q];
+    step[i] = x;
+  }
+#pragma unroll
+  for (int i = 0; i < cdm_np; ++i) step[i] /= dg[i];  // D z = y
+#pragma unroll
+  for (int i = cdm_np - 1; i >= 0; --i) {  // L^T x = z
+    double x = step[i];
+#pragma unroll
+    for (int q = i + 1; q < cdm_np; ++q) x -= fac[q * (q + 1) / 2 + i] * step[q];
+    step[i] = x;
+  }
+  // the next factorisation's stores must not pass these reads
+  cdm_wave_lds_sync();
 }
 
 // The kernel bodies take the block index as an argument so that the batched
@@ -767,31 +961,53 @@ __device__ __forceinline__ void cdm_levmar_voxel(const cdm_lm_args& a, int marqu
 #pragma unroll
       for (int k = 0; k < cdm_ntri; ++k) hs[k] = 0;
       cdm_cfp_row<cdm_jt> rowj;
-      cdm_jacobian_setup<cdm_jt> jac;
-      jac.template init<L>(pj, fxj, lane);
       if constexpr (cdm_lm_lds_normal<L>) {
         CDM_F64* rows = reinterpret_cast<CDM_F64*>(reinterpret_cast<char*>(s_cfp) +
                                                    cdm_lm_scratch_offset(a.g.nmeas, a.g.cfp_ts)) +
                         (size_t)(threadIdx.x / L) * cdm_lm_scratch_doubles(L, a.g.nmeas);
         CDM_F64* sums = rows + (size_t)a.g.nmeas * (cdm_np + 1);
-        for (int m = lane; m < a.g.nmeas; m += L) {
-          cdm_jt* cm = rowj.at(s_cfp, m, a.g.cfp_ts);
-          cdm_jt d[cdm_np];
-          const cdm_jt r = jac.eval(pj, cm, fxj, cdm_mc_row(s_mc, m), d) - (cdm_jt)mv[m];
+        if constexpr (cdm_lm_columns) {
+          cdm_lm_jacobian_columns<L>(a.g, s_cfp, s_mc, mv, pj, fxj, dtf, rows, lane);
+        } else {
+          cdm_jacobian_setup<cdm_jt> jac;
+          jac.template init<L>(pj, fxj, lane);
+          for (int m = lane; m < a.g.nmeas; m += L) {
+            cdm_jt* cm = rowj.at(s_cfp, m, a.g.cfp_ts);
+            cdm_jt d[cdm_np];
+            const cdm_jt r = jac.eval(pj, cm, fxj, cdm_mc_row(s_mc, m), d) - (cdm_jt)mv[m];
 #pragma unroll
-          for (int i = 0; i < cdm_np; ++i) rows[m * (cdm_np + 1) + i] = (CDM_F64)(cdm_lm_acc)(d[i] * dtf[i]);
-          rows[m * (cdm_np + 1) + cdm_np] = (CDM_F64)(cdm_lm_acc)r;
+            for (int i = 0; i < cdm_np; ++i) rows[m * (cdm_np + 1) + i] = (CDM_F64)(cdm_lm_acc)(d[i] * dtf[i]);
+            rows[m * (cdm_np + 1) + cdm_np] = (CDM_F64)(cdm_lm_acc)r;
+          }
         }
         cdm_wave_lds_sync();
         cdm_lds_normal_sums<L>(rows, sums, a.g.nmeas, lane);
         cdm_wave_lds_sync();
 #pragma unroll
         for (int i = 0; i < cdm_np; ++i) gl[i] = (cdm_lm_acc)sums[cdm_ntri + i];
+        // 2 J^T J in place (fixed parameters: identity rows and columns),
+        // kept in LDS for this iteration's damped solves
+        for (int e = lane; e < cdm_ntri; e += L) {
+          int i = 0, j = 0;
+          cdm::tri_ij<cdm_np>(e, i, j);
+          int fi = 0, fj = 0;
 #pragma unroll
-        for (int k = 0; k < cdm_ntri; ++k) hs[k] = (cdm_lm_acc)sums[k];
-        // the next iteration's stores into `rows` must not pass this one's reads
+          for (int k = 0; k < cdm_np; ++k) {
+            if (k == i) fi = a.c.fixed[k];
+            if (k == j) fj = a.c.fixed[k];
+          }
+          cdm_lm_acc h = (cdm_lm_acc)2 * (cdm_lm_acc)sums[e];
+          if (fi || fj) h = (i == j) ? (cdm_lm_acc)1 : (cdm_lm_acc)0;
+          sums[e] = (CDM_F64)h;
+        }
+        // the next iteration's stores into `rows` must not pass this one's
+        // reads, and every lane sees the doubled matrix
         cdm_wave_lds_sync();
+#pragma unroll
+        for (int i = 0; i < cdm_np; ++i) diag[i] = (double)(cdm_lm_acc)sums[i * (i + 1) / 2 + i];
       } else {
+        cdm_jacobian_setup<cdm_jt> jac;
+        jac.template init<L>(pj, fxj, lane);
         for (int m = lane; m < a.g.nmeas; m += L) {
           cdm_jt* cm = rowj.at(s_cfp, m, a.g.cfp_ts);
           cdm_jt d[cdm_np];
@@ -815,17 +1031,19 @@ __device__ __forceinline__ void cdm_levmar_voxel(const cdm_lm_args& a, int marqu
       }
 #pragma unroll
       for (int i = 0; i < cdm_np; ++i) grad[i] = a.c.fixed[i] ? 0.0 : 2.0 * (double)gl[i];
-      int k = 0;
+      if constexpr (!cdm_lm_lds_normal<L>) {
+        int k = 0;
 #pragma unroll
-      for (int i = 0; i < cdm_np; ++i)
+        for (int i = 0; i < cdm_np; ++i)
 #pragma unroll
-        for (int j = 0; j <= i; ++j, ++k) {
-          cdm_lm_acc h = (cdm_lm_acc)2 * hs[k];
-          if (a.c.fixed[i] || a.c.fixed[j]) h = (i == j) ? (cdm_lm_acc)1 : (cdm_lm_acc)0;
-          hs[k] = h;
-        }
+          for (int j = 0; j <= i; ++j, ++k) {
+            cdm_lm_acc h = (cdm_lm_acc)2 * hs[k];
+            if (a.c.fixed[i] || a.c.fixed[j]) h = (i == j) ? (cdm_lm_acc)1 : (cdm_lm_acc)0;
+            hs[k] = h;
+          }
 #pragma unroll
-      for (int i = 0; i < cdm_np; ++i) diag[i] = hs[i * (i + 1) / 2 + i];
+        for (int i = 0; i < cdm_np; ++i) diag[i] = hs[i * (i + 1) / 2 + i];
+      }
     }
     // Damping applied cumulatively to the kept diagonal, as the reference
     // damps its shared Hessian in place (Levenberg_Marquardt.cu:530-537).
@@ -839,7 +1057,15 @@ __device__ __forceinline__ void cdm_levmar_voxel(const cdm_lm_args& a, int marqu
     // pivoting, so like the reference's Gauss-Jordan solve it gives a step
     // for any matrix with non-zero pivots, and inf/NaN (a rejected step)
     // for a zero pivot (Levenberg_Marquardt.cu:336-407).
-    double fac[cdm_ntri], dg[cdm_np];
+    double step[cdm_np], dg[cdm_np];
+    if constexpr (cdm_lm_lds_normal<L>) {
+      CDM_F64* rows = reinterpret_cast<CDM_F64*>(reinterpret_cast<char*>(s_cfp) +
+                                                 cdm_lm_scratch_offset(a.g.nmeas, a.g.cfp_ts)) +
+                      (size_t)(threadIdx.x / L) * cdm_lm_scratch_doubles(L, a.g.nmeas);
+      const CDM_F64* hsum = rows + (size_t)a.g.nmeas * (cdm_np + 1);
+      cdm_lds_damped_solve<L>(hsum, const_cast<CDM_F64*>(hsum) + cdm_ntri + cdm_np, diag, grad, step, dg, lane);
+    } else {
+    double fac[cdm_ntri];
     {
       int k = 0;
 #pragma unroll
@@ -862,7 +1088,6 @@ __device__ __forceinline__ void cdm_levmar_voxel(const cdm_lm_args& a, int marqu
         }
       }
     }
-    double step[cdm_np];
 #pragma unroll
     for (int i = 0; i < cdm_np; ++i) {  // L y = g
       double x = grad[i];
@@ -878,6 +1103,7 @@ __device__ __forceinline__ void cdm_levmar_voxel(const cdm_lm_args& a, int marqu
 #pragma unroll
       for (int q = i + 1; q < cdm_np; ++q) x -= fac[q * (q + 1) / 2 + i] * step[q];
       step[i] = x;
+    }
     }
     double tn[cdm_np];
 #pragma unroll
@@ -1001,10 +1227,29 @@ __host__ __device__ __forceinline__ size_t cdm_vslot_offset(int nmeas, int cfp_t
   return (b + 15) / 16 * 16;
 }
 
+// The two per-measurement cache rows of each voxel slot in the dynamic LDS,
+// after the slots: row_stride values apart (a row and its padding), 2 rows
+// per slot.  Rows are rebuilt at the start of every launch, so they need
+// not outlive it.
+__host__ __device__ __forceinline__ size_t cdm_mcmc_rows_offset(int nmeas, int cfp_ts, int L) {
+  const size_t b = cdm_vslot_offset(nmeas, cfp_ts) + (size_t)(cdm_block / L) * sizeof(cdm_vslot);
+  return (b + 15) / 16 * 16;
+}
+
+// Values per LDS cache row: at least nmeas * cdm_mvc, and = 2 mod 16, so
+// that the 16 four-lane groups of a wave (slots two rows apart) start on
+// different banks.
+__host__ __device__ __forceinline__ int cdm_mcmc_row_stride(int nmeas) {
+  const int n = nmeas * (cdm_mvc > 0 ? cdm_mvc : 1);
+  return n + ((2 - n % 16) + 16) % 16;
+}
+
 struct cdm_mcmc_args {
   cdm_geom g;
   float* params;         // [nvox][NP] in/out
-  cdm_tf* mvc;           // [nvox][2][nmeas][cdm_mvc] per-measurement caches (cdm_mvc > 0)
+  cdm_tf* mvc;           // [nvox][2][nmeas][cdm_mvc] per-measurement caches (cdm_mvc > 0, rows_lds == 0)
+  int rows_lds;          // the cache rows live in LDS (cdm_mcmc_rows_offset) instead of mvc
+  int row_stride;        // values between an LDS slot's two rows
   cdm_chain* state;      // [nvox]
   float* samples;        // [nvox][NP][nsamples] (recording)
   float* tau_samples;    // [nvox][nsamples] (Rician, recording)
@@ -1245,7 +1490,7 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
   // are kept too, so that a step in a parameter the jobs do not depend on
   // only recombines them
 #if defined(CDM_VOXEL_CACHE) && defined(CDM_VOXEL_CACHE_JOBS)
-  constexpr bool job_split = L > 1 && cdm_vcj_dep != cdm_vc_dep;
+  constexpr bool job_split = cdm_vcj_dep != cdm_vc_dep;
 #else
   constexpr bool job_split = false;
 #endif
@@ -1262,16 +1507,64 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
   } else {
     cdm_prepare_dg<L>(p, fx, vc_cur, lane);
   }
-  // per-measurement caches: two rows per voxel, `cur` holds the current state's
-  cdm_tf* mvc_rows = cdm_mvc ? a.mvc + (size_t)v * 2 * a.g.nmeas * cdm_mvc : nullptr;
+  // per-measurement caches: two rows per voxel, `cur` holds the current
+  // state's; in the block's LDS when they fit there, else in global memory.
+  // Each lane reads only the entries it wrote itself, so neither needs a
+  // barrier; row 0 is filled at the start of every launch.
+  cdm_tf* mvc_rows = nullptr;
+  size_t rs = 0;
+  if constexpr (cdm_mvc > 0) {
+    if (a.rows_lds) {
+      rs = (size_t)a.row_stride;
+      mvc_rows = reinterpret_cast<cdm_tf*>(reinterpret_cast<char*>(s_cfp) +
+                                           cdm_mcmc_rows_offset(a.g.nmeas, a.g.cfp_ts, L)) +
+                 (size_t)(threadIdx.x / L) * 2 * rs;
+    } else {
+      rs = (size_t)a.g.nmeas * cdm_mvc;
+      mvc_rows = a.mvc + (size_t)v * 2 * rs;
+    }
+  }
   int cur = 0;
   if constexpr (cdm_mvc > 0)
     (void)cdm_neg_loglik_mv<L, RICIAN, 2>(a.g, s_cfp, s_mc, mv, p, tau, fx, vc_cur, mvc_rows, lane);
   const int end = a.begin + a.count;
   for (int it = a.begin; it < end; ++it) {
     const uint32_t jump = a.jump_offset + (uint32_t)it;
+    // The jump's draws do not depend on the chain, so the group computes them
+    // up front, slot s on lane s % L (round s / L), and a step takes its slot's
+    // pair from that lane: each lane runs ceil(slots / L) Philox blocks and
+    // Box-Muller transforms instead of all of them.  Same values as drawing
+    // slot by slot.
+    // (One lane per voxel draws each slot where it is used.)
+    constexpr int nslots = cdm_np + (RICIAN ? 1 : 0);
+    constexpr int rounds = L > 1 ? (nslots + L - 1) / L : 1;
+    double dz[rounds], du[rounds];
+    if constexpr (L > 1) {
+#pragma unroll
+      for (int r = 0; r < rounds; ++r) {
+        int slot = lane + r * L;
+        slot = slot < nslots ? slot : nslots - 1;
+        const cdm::McmcDraw d = cdm::mcmc_draw(a.seed, gvox, jump, (uint32_t)slot);
+        dz[r] = d.normal;
+        du[r] = d.uniform;
+      }
+    }
+    auto draw_of = [&](int slot) {
+      if constexpr (L == 1) return cdm::mcmc_draw(a.seed, gvox, jump, (uint32_t)slot);
+      double z = dz[0], u = du[0];
+#pragma unroll
+      for (int r = 1; r < rounds; ++r)
+        if (slot / L == r) {
+          z = dz[r];
+          u = du[r];
+        }
+      cdm::McmcDraw d;
+      d.normal = cdm::Group<L>::bcast(z, slot % L);
+      d.uniform = cdm::Group<L>::bcast(u, slot % L);
+      return d;
+    };
     if constexpr (RICIAN) {
-      const cdm::McmcDraw d = cdm::mcmc_draw(a.seed, gvox, jump, (uint32_t)cdm_np);
+      const cdm::McmcDraw d = draw_of(cdm_np);
       const double old = tau;
       tau = tau + d.normal * tau_sd;
       if (tau > 0.0) {
@@ -1280,7 +1573,7 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
         for (int i = 0; i < cdm_np; ++i) tp += prior[i];
         const double e_new =
             tp + (cdm_mvc > 0 ? cdm_neg_loglik_mv<L, true, 1>(a.g, s_cfp, s_mc, mv, p, tau, fx, vc_cur,
-                                                              mvc_rows + cur * a.g.nmeas * cdm_mvc, lane)
+                                                              mvc_rows + cur * rs, lane)
                               : cdm_neg_loglik<L, true>(a.g, s_cfp, s_mc, mv, p, tau, fx, vc_cur, lane));
         if (exp(energy - e_new) > d.uniform) {
           ++tau_acc;
@@ -1296,7 +1589,7 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
     }
 #pragma unroll 1
     for (int i = 0; i < cdm_np; ++i) {
-      const cdm::McmcDraw d = cdm::mcmc_draw(a.seed, gvox, jump, (uint32_t)i);
+      const cdm::McmcDraw d = draw_of(i);
       const double old = p[i];
       if (!a.c.fixed[i]) p[i] = p[i] + d.normal * sd[i];
       bool ok = ConstraintsMCMC(cdm_np, p);
@@ -1335,7 +1628,6 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
         for (int k = 0; k < cdm_vc; ++k) vcu[k] = dep ? vc_new[k] : vc_cur[k];
         double like;
         if constexpr (cdm_mvc > 0) {
-          const size_t rs = (size_t)a.g.nmeas * cdm_mvc;
           like = mdep ? cdm_neg_loglik_mv<L, RICIAN, 2>(a.g, s_cfp, s_mc, mv, p, tau, fx, vcu,
                                                         mvc_rows + (1 - cur) * rs, lane)
                       : cdm_neg_loglik_mv<L, RICIAN, 1>(a.g, s_cfp, s_mc, mv, p, tau, fx, vcu, mvc_rows + cur * rs,

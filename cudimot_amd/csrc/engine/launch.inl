@@ -73,8 +73,8 @@ static hipError_t op_levmar(int L, int W, const cdm::Geom& g, float* params, int
                             const cdm::Consts& c, int* iters, float* cost, int* queue, int ncu, hipStream_t s) {
   if (g.nvox <= 0) return hipSuccess;
   if (g.cfp_ts > cdm_cfp_max || g.fixp_ts > cdm_fixp_max) return hipErrorInvalidValue;
-  // 64-lane groups keep the rows of J in LDS; past the LDS of a CU, 16 lanes
-  if (L == 64 && op_lds_lm(g, 64) > cdm_lds_max) L = 16;
+  L = op_lm_lanes(g, L);
+  if (op_lds_lm(g, L) > cdm_lds_max) return hipErrorInvalidValue;
   cdm_lm_args a = op_lm_args(g, params, max_iters, c, iters, cost);
   if (!queue) {
     CDM_DISPATCH_L(L, CDM_DISPATCH_W(W, hipLaunchKernelGGL((cdm_levmar_kernel<LG, WG>), op_grid(g.nvox, LG),
@@ -92,7 +92,7 @@ static hipError_t op_mcmc(int L, int W, int rician, const cdm::Geom& g, const cd
                           const cdm::Consts& c, hipStream_t s) {
   if (g.nvox <= 0) return hipSuccess;
   if (g.cfp_ts > cdm_cfp_max || g.fixp_ts > cdm_fixp_max) return hipErrorInvalidValue;
-  const cdm_mcmc_args a = op_mcmc_args(g, m, c);
+  const cdm_mcmc_args a = op_mcmc_args(g, L, m, c);
   if (rician) {  // the Rician likelihood is compiled at the default register budget only
     CDM_DISPATCH_L(L, hipLaunchKernelGGL((cdm_mcmc_kernel<LG, true, 1>), op_grid(g.nvox, LG), dim3(cdm_block),
                                          op_lds_mcmc(g, LG), s, a));
@@ -102,6 +102,8 @@ static hipError_t op_mcmc(int L, int W, int rician, const cdm::Geom& g, const cd
   }
   return hipGetLastError();
 }
+
+static int op_mcmc_rows_lds(int L, const cdm::Geom& g) { return op_mcmc_rows_lds_bytes(g, L) > 0 ? 1 : 0; }
 
 static hipError_t op_predicted(int L, const cdm::Geom& g, int nsamples, const float* samples, float* out,
                                hipStream_t s) {

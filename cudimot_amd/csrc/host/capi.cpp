@@ -10,13 +10,36 @@
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
 
 #include "../../../include/cudimot_amd.h"
+#include "fill.h"
 #include "model_ops.h"
 #include "spec.h"
+
+namespace {
+
+// A captured launch sequence, kept instantiated and replayed while its key
+// (every value the launches bake into the graph's nodes) is unchanged.
+struct GraphCache {
+  std::vector<unsigned char> key;
+  hipGraphExec_t exec = nullptr;
+  void clear() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    key.clear();
+  }
+};
+
+// Counters of the graph caches (cdm_graph_stats).
+struct GraphStats {
+  long long captures = 0, replays = 0;
+};
+
+}  // namespace
 
 struct cdm_model {
   const cdm::ModelOps* ops = nullptr;
@@ -43,6 +66,17 @@ struct cdm_model {
   size_t state_bytes = 0;
   // LM work-queue counter (one int; calls on one handle are serialised)
   int* d_queue = nullptr;
+  // MCMC phases recorded into HIP graphs (jump blocks with use_graph), one
+  // per phase, and the private stream they are captured on
+  GraphCache mcmc_graphs[2];
+  hipStream_t capture_stream = nullptr;
+  GraphStats graph_stats;
+  // the stream of the handle's last launch and an event after it: a call on
+  // another stream waits for it, since the queue counter and the chain
+  // state belong to the handle
+  hipStream_t last_stream = nullptr;
+  hipEvent_t last_done = nullptr;
+  bool has_last = false;
 };
 
 namespace {
@@ -276,8 +310,8 @@ size_t mvc_offset(const cdm_model* m, int nvox) {
   return (m->ops->chain_bytes * (size_t)nvox + 255) / 256 * 256;
 }
 
-int ensure_state(cdm_model* m, int nvox, int nmeas) {
-  const size_t bytes = mvc_offset(m, nvox) + m->ops->mvc_bytes_per_meas * 2 * (size_t)nmeas * (size_t)nvox;
+// The handle's chain-state buffer, grown to at least `bytes`.
+int ensure_state(cdm_model* m, size_t bytes) {
   if (m->state_bytes >= bytes) return 0;
   if (m->d_state) (void)hipFree(m->d_state);
   m->d_state = nullptr;
@@ -292,28 +326,75 @@ void* mvc_rows(cdm_model* m, int nvox) {
   return m->ops->mvc_bytes_per_meas ? (void*)((char*)m->d_state + mvc_offset(m, nvox)) : nullptr;
 }
 
-// Records the launches that `issue` makes on a private capture stream into a
-// HIP graph and replays it once on `s` (a capture needs a stream other than
-// the null stream; the replay keeps the caller's stream order).
+// Replays the graph of `gc` on `s` when its key equals `key`; otherwise
+// records the launches that `issue` makes on the private capture stream `*cs`
+// (created on first use: a capture needs a stream other than the null
+// stream) into a new graph, instantiates it, keeps it with `key` and launches
+// it on `s`, which keeps the caller's stream order.
 template <class F>
-hipError_t run_as_graph(hipStream_t s, F issue) {
-  hipStream_t cs = nullptr;
-  hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
-  if (e != hipSuccess) return e;
-  e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-  if (e == hipSuccess) {
-    const hipError_t ei = issue(cs);
-    hipGraph_t g = nullptr;
-    const hipError_t ec = hipStreamEndCapture(cs, &g);
-    e = ei != hipSuccess ? ei : ec;
-    hipGraphExec_t exec = nullptr;
-    if (e == hipSuccess) e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-    if (e == hipSuccess) e = hipGraphLaunch(exec, s);
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (g) (void)hipGraphDestroy(g);
+hipError_t run_graph_cached(GraphCache& gc, hipStream_t* cs, GraphStats* st, const std::vector<unsigned char>& key,
+                            hipStream_t s, F issue) {
+  if (gc.exec && gc.key == key) {
+    ++st->replays;
+    return hipGraphLaunch(gc.exec, s);
   }
-  (void)hipStreamDestroy(cs);
-  return e;
+  gc.clear();
+  hipError_t e = hipSuccess;
+  if (!*cs) e = hipStreamCreateWithFlags(cs, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    *cs = nullptr;
+    return e;
+  }
+  e = hipStreamBeginCapture(*cs, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  const hipError_t ei = issue(*cs);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(*cs, &g);
+  e = ei != hipSuccess ? ei : ec;
+  if (e == hipSuccess) e = hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    gc.exec = nullptr;
+    return e;
+  }
+  gc.key = key;
+  ++st->captures;
+  return hipGraphLaunch(gc.exec, s);
+}
+
+// Appends the bytes of a value to a graph key.
+template <class V>
+void key_add(std::vector<unsigned char>* key, const V& v) {
+  const unsigned char* p = reinterpret_cast<const unsigned char*>(&v);
+  key->insert(key->end(), p, p + sizeof(V));
+}
+
+// Orders a call of handle m on stream s after the handle's last launch when
+// that was on another stream (the work-queue counter and the chain state
+// belong to the handle).  Skipped while s is being captured: a graph cannot
+// wait on work outside it, and the capturing caller orders its streams.
+hipError_t order_after_last(cdm_model* m, hipStream_t s) {
+  if (!m->has_last || m->last_stream == s) return hipSuccess;
+  hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
+  hipError_t e = hipStreamIsCapturing(s, &cst);
+  if (e != hipSuccess || cst != hipStreamCaptureStatusNone) return e;
+  return hipStreamWaitEvent(s, m->last_done, 0);
+}
+
+// Records the end of a call of handle m on stream s (not while s is being
+// captured).
+hipError_t note_last(cdm_model* m, hipStream_t s) {
+  hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
+  hipError_t e = hipStreamIsCapturing(s, &cst);
+  if (e != hipSuccess || cst != hipStreamCaptureStatusNone) return e;
+  if (!m->last_done && (e = hipEventCreateWithFlags(&m->last_done, hipEventDisableTiming)) != hipSuccess) {
+    m->last_done = nullptr;
+    return e;
+  }
+  if ((e = hipEventRecord(m->last_done, s)) != hipSuccess) return e;
+  m->last_stream = s;
+  m->has_last = true;
+  return hipSuccess;
 }
 
 }  // namespace
@@ -370,10 +451,21 @@ int cdm_model_open(const char* name, const char* priors_file, cdm_model** out) {
 
 void cdm_model_close(cdm_model* m) {
   if (!m) return;
+  if (m->has_last && m->last_done) (void)hipEventSynchronize(m->last_done);
   free_grid(m);
   if (m->d_state) (void)hipFree(m->d_state);
   if (m->d_queue) (void)hipFree(m->d_queue);
+  for (GraphCache& g : m->mcmc_graphs) g.clear();
+  if (m->capture_stream) (void)hipStreamDestroy(m->capture_stream);
+  if (m->last_done) (void)hipEventDestroy(m->last_done);
   delete m;
+}
+
+int cdm_model_graph_stats(const cdm_model* m, long long* captures, long long* replays) {
+  if (!m) return fail("null model handle");
+  if (captures) *captures = m->graph_stats.captures;
+  if (replays) *replays = m->graph_stats.replays;
+  return 0;
 }
 
 int cdm_model_info(const cdm_model* m, int* nparams, int* cfp_tsize, int* fixp_tsize) {
@@ -514,15 +606,23 @@ int cdm_levmar(cdm_model* m, int nvox, int nmeas, const float* meas, float* para
   if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) !=
                                               hipSuccess)
     ncu = 0;
-  const hipError_t e = m->ops->levmar(group_for(m, cdm::kLevMar, nmeas, nvox), occ_for(m, cdm::kLevMar, nmeas), g,
-                                      params, max_iters, marquardt ? 1 : 0, make_consts(m), iters, cost, m->d_queue,
-                                      ncu, (hipStream_t)stream);
-  return e == hipSuccess ? 0 : hip_fail(e, "Levenberg-Marquardt launch");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = order_after_last(m, s);
+  if (e != hipSuccess) return hip_fail(e, "Levenberg-Marquardt stream order");
+  e = m->ops->levmar(group_for(m, cdm::kLevMar, nmeas, nvox), occ_for(m, cdm::kLevMar, nmeas), g, params, max_iters,
+                     marquardt ? 1 : 0, make_consts(m), iters, cost, m->d_queue, ncu, s);
+  if (e != hipSuccess) return hip_fail(e, "Levenberg-Marquardt launch");
+  e = note_last(m, s);
+  return e == hipSuccess ? 0 : hip_fail(e, "Levenberg-Marquardt stream order");
 }
 
-int cdm_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params, const float* cfp,
-             const float* fixp, int nburn, int njumps, int sampleevery, int update_proposal, int rician,
-             unsigned long long seed, long long voxel_offset, float* samples, float* tau_samples, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// Argument checks of cdm_mcmc (no device work).
+int check_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, const float* cfp, const float* fixp, int njumps,
+               int sampleevery, int rician, const float* samples, const float* tau_samples) {
   int r = check_geom(m, nvox, nmeas, meas, cfp, fixp, true);
   if (r) return r;
   if (sampleevery <= 0) return fail("--sampleevery must be > 0");
@@ -531,22 +631,40 @@ int cdm_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params
                 "greater than or equal to space between samples (--se)");
   if (!samples && nvox > 0) return fail("samples pointer is null");
   if (rician && !tau_samples && nvox > 0) return fail("--rician needs tau_samples");
-  if (nvox == 0) return 0;
-  if (njumps % sampleevery != 0)
+  if (nvox > 0 && njumps % sampleevery != 0)
     return fail("--nj must be a multiple of --se (the reference writes past the sample buffer otherwise)");
+  return 0;
+}
+
+// Bytes of chain state an MCMC call over nvox voxels needs: the chains, then
+// the per-measurement cache rows unless the launch keeps them in LDS.
+size_t mcmc_state_bytes(const cdm_model* m, int nvox, int nmeas, const float* cfp, const float* fixp) {
+  const int L = group_for(m, cdm::kMcmc, nmeas, nvox);
+  const cdm::Geom g = make_geom(m, nvox, nmeas, nullptr, cfp, fixp);
+  if (!m->ops->mvc_bytes_per_meas || m->ops->mcmc_rows_lds(L, g)) return m->ops->chain_bytes * (size_t)nvox;
+  return mvc_offset(m, nvox) + m->ops->mvc_bytes_per_meas * 2 * (size_t)nmeas * (size_t)nvox;
+}
+
+// Burn-in then recording, each phase as one launch or as jump blocks
+// (cdm_model_set_mcmc_blocks), over the chain state at `state` (nvox voxels,
+// their cache rows after it).  allow_graph: record a blocked phase into a
+// HIP graph when the handle asks for it.
+int run_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params, const float* cfp, const float* fixp,
+             int nburn, int njumps, int sampleevery, int update_proposal, int rician, unsigned long long seed,
+             long long voxel_offset, float* samples, float* tau_samples, void* state, bool allow_graph,
+             hipStream_t s) {
   const int nb = nburn < 1 ? 1 : nburn;  // MCMC.cu:702-703
   const int nsamples = njumps / sampleevery;
-  if ((r = ensure_state(m, nvox, nmeas))) return r;
   const cdm::Geom g = make_geom(m, nvox, nmeas, meas, cfp, fixp);
   const cdm::Consts c = make_consts(m);
   const int L = group_for(m, cdm::kMcmc, nmeas, nvox);
   const int W = occ_for(m, cdm::kMcmc, nmeas);
-  hipStream_t s = (hipStream_t)stream;
 
   cdm::McmcLaunch ml;
   ml.params = params;
-  ml.state = m->d_state;
-  ml.mvc = mvc_rows(m, nvox);
+  ml.state = state;
+  ml.mvc = (m->ops->mvc_bytes_per_meas && !m->ops->mcmc_rows_lds(L, g)) ? (void*)((char*)state + mvc_offset(m, nvox))
+                                                                       : nullptr;
   ml.samples = samples;
   ml.tau_samples = tau_samples;
   ml.nsamples = nsamples;
@@ -570,10 +688,45 @@ int cdm_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params
       }
       return hipSuccess;
     };
-    const hipError_t e = (m->mcmc_graph && len > blk) ? run_as_graph(s, issue) : issue(s);
+    hipError_t e;
+    if (allow_graph && m->mcmc_graph && len > blk) {
+      // the phase's graph is replayed while every argument its launches
+      // bake in is the same (the jump-block offsets follow from len, blk)
+      std::vector<unsigned char> key;
+      cdm::McmcLaunch mk = ml;
+      mk.begin = mk.count = 0;
+      key_add(&key, g);
+      key_add(&key, c);
+      key_add(&key, mk);
+      const int shape[5] = {L, W, rician, blk, len};
+      key_add(&key, shape);
+      e = run_graph_cached(m->mcmc_graphs[phase], &m->capture_stream, &m->graph_stats, key, s, issue);
+    } else {
+      e = issue(s);
+    }
     if (e != hipSuccess) return hip_fail(e, "MCMC launch");
   }
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cdm_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params, const float* cfp,
+             const float* fixp, int nburn, int njumps, int sampleevery, int update_proposal, int rician,
+             unsigned long long seed, long long voxel_offset, float* samples, float* tau_samples, void* stream) {
+  int r = check_mcmc(m, nvox, nmeas, meas, cfp, fixp, njumps, sampleevery, rician, samples, tau_samples);
+  if (r || nvox == 0) return r;
+  if ((r = ensure_state(m, mcmc_state_bytes(m, nvox, nmeas, cfp, fixp)))) return r;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = order_after_last(m, s);
+  if (e != hipSuccess) return hip_fail(e, "MCMC stream order");
+  r = run_mcmc(m, nvox, nmeas, meas, params, cfp, fixp, nburn, njumps, sampleevery, update_proposal, rician, seed,
+               voxel_offset, samples, tau_samples, m->d_state, true, s);
+  if (r) return r;
+  e = note_last(m, s);
+  return e == hipSuccess ? 0 : hip_fail(e, "MCMC stream order");
 }
 
 int cdm_predicted_signal(cdm_model* m, int nvox, int nmeas, int nsamples, const float* samples, const float* cfp,
@@ -710,14 +863,92 @@ std::vector<int> batch_order(int nitems, const cdm_batch_item* items, int method
   return order;
 }
 
-struct BatchBuffers {
-  void* items = nullptr;
-  int* queues = nullptr;
-  ~BatchBuffers() {
-    if (items) (void)hipFree(items);
-    if (queues) (void)hipFree(queues);
-  }
+// Device item table, work-queue counters and pinned host staging of the
+// batched launches, one set per (method, device), grown as needed and kept
+// for the process.  A call first waits (on the host) for the previous call on
+// the same set, then rewrites it; its own launches stay asynchronous on the
+// caller's stream.  Calls are serialised by g_batch_mutex.
+struct BatchWorkspace {
+  void* d_items = nullptr;
+  size_t items_cap = 0;
+  int* d_queues = nullptr;
+  size_t queues_cap = 0;
+  unsigned char* h_stage = nullptr;
+  size_t stage_cap = 0;
+  hipEvent_t done = nullptr;
+  bool pending = false;
+  GraphCache graphs[2];  // per MCMC phase
+  hipStream_t capture_stream = nullptr;
 };
+
+constexpr int kBatchDevices = 64;
+std::mutex g_batch_mutex;
+BatchWorkspace g_batch[2][kBatchDevices];
+GraphStats g_batch_stats;
+
+hipError_t batch_acquire(int mode, size_t item_bytes, size_t nqueues, BatchWorkspace** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kBatchDevices) return hipErrorInvalidDevice;
+  BatchWorkspace& w = g_batch[mode][dev];
+  if (w.pending && (e = hipEventSynchronize(w.done)) != hipSuccess) return e;
+  w.pending = false;
+  if (!w.done && (e = hipEventCreateWithFlags(&w.done, hipEventDisableTiming)) != hipSuccess) {
+    w.done = nullptr;
+    return e;
+  }
+  if (w.items_cap < item_bytes) {
+    if (w.d_items) (void)hipFree(w.d_items);
+    w.d_items = nullptr;
+    w.items_cap = 0;
+    if ((e = hipMalloc(&w.d_items, item_bytes)) != hipSuccess) return e;
+    w.items_cap = item_bytes;
+  }
+  if (w.stage_cap < item_bytes) {
+    if (w.h_stage) (void)hipHostFree(w.h_stage);
+    w.h_stage = nullptr;
+    w.stage_cap = 0;
+    if ((e = hipHostMalloc((void**)&w.h_stage, item_bytes, hipHostMallocDefault)) != hipSuccess) return e;
+    w.stage_cap = item_bytes;
+  }
+  if (w.queues_cap < nqueues) {
+    if (w.d_queues) (void)hipFree(w.d_queues);
+    w.d_queues = nullptr;
+    w.queues_cap = 0;
+    if ((e = hipMalloc((void**)&w.d_queues, nqueues * sizeof(int))) != hipSuccess) return e;
+    w.queues_cap = nqueues;
+  }
+  *out = &w;
+  return hipSuccess;
+}
+
+// Stages the prepared items into the workspace and copies them to the device.
+hipError_t batch_upload(BatchWorkspace* w, const std::vector<unsigned char>& host, hipStream_t s) {
+  std::memcpy(w->h_stage, host.data(), host.size());
+  return hipMemcpyAsync(w->d_items, w->h_stage, host.size(), hipMemcpyHostToDevice, s);
+}
+
+// Orders the items' handles' work on s and records its end on them.
+hipError_t batch_order(int nitems, const cdm_batch_item* items, hipStream_t s) {
+  for (int i = 0; i < nitems; ++i) {
+    const hipError_t e = order_after_last(items[i].model, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+hipError_t batch_note(int nitems, const cdm_batch_item* items, BatchWorkspace* w, hipStream_t s) {
+  for (int i = 0; i < nitems; ++i) {
+    const hipError_t e = note_last(items[i].model, s);
+    if (e != hipSuccess) return e;
+  }
+  hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
+  hipError_t e = hipStreamIsCapturing(s, &cst);
+  if (e != hipSuccess || cst != hipStreamCaptureStatusNone) return e;
+  if ((e = hipEventRecord(w->done, s)) != hipSuccess) return e;
+  w->pending = true;
+  return hipSuccess;
+}
 
 }  // namespace
 
@@ -749,16 +980,17 @@ int cdm_batch_levmar(int nitems, const cdm_batch_item* items, int max_iters, int
   size_t lds = 0;
   if (unit->prepare(cdm::kBatchLevMar, nitems, li.data(), host.data(), &tiles, &lds))
     return fail("batch LM: an item does not fit the engine limits");
-  BatchBuffers buf;
-  hipError_t e = hipMalloc(&buf.items, host.size());
-  if (e == hipSuccess) e = hipMalloc((void**)&buf.queues, sizeof(int));
-  if (e == hipSuccess) e = hipMemcpyAsync(buf.items, host.data(), host.size(), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemsetAsync(buf.queues, 0, sizeof(int), s);
+  std::lock_guard<std::mutex> lock(g_batch_mutex);
+  BatchWorkspace* w = nullptr;
+  hipError_t e = batch_acquire(cdm::kBatchLevMar, host.size(), 1, &w);
+  if (e == hipSuccess) e = batch_order(nitems, items, s);
+  if (e == hipSuccess) e = batch_upload(w, host, s);
+  if (e == hipSuccess) e = cdm::zero_ints(w->d_queues, 1, s);
   int grid = 0;
   if (e == hipSuccess) e = unit->grid(cdm::kBatchLevMar, tiles, lds, &grid);
   if (e == hipSuccess)
-    e = unit->launch(cdm::kBatchLevMar, buf.items, nitems, tiles, buf.queues, lds, marquardt ? 1 : 0, grid, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
+    e = unit->launch(cdm::kBatchLevMar, w->d_items, nitems, tiles, w->d_queues, lds, marquardt ? 1 : 0, grid, s);
+  if (e == hipSuccess) e = batch_note(nitems, items, w, s);
   return e == hipSuccess ? 0 : hip_fail(e, "batch LM");
 }
 
@@ -778,8 +1010,12 @@ int cdm_batch_mcmc(int nitems, const cdm_batch_item* items, int nburn, int njump
   const int nb = nburn < 1 ? 1 : nburn;  // MCMC.cu:702-703
   const int nsamples = njumps / sampleevery;
   hipStream_t s = (hipStream_t)stream;
-  for (int i = 0; i < nitems; ++i)
-    if ((r = ensure_state(items[i].model, items[i].nvox, items[i].nmeas))) return r;
+  for (int i = 0; i < nitems; ++i) {
+    const cdm_model* im = items[i].model;
+    const size_t bytes = mvc_offset(im, items[i].nvox) +
+                         im->ops->mvc_bytes_per_meas * 2 * (size_t)items[i].nmeas * (size_t)items[i].nvox;
+    if ((r = ensure_state(items[i].model, bytes))) return r;
+  }
   const std::vector<int> order = batch_order(nitems, items, cdm::kMcmc);
   // one prepared item set per launch: phases x jump blocks
   struct Launch {
@@ -827,11 +1063,11 @@ int cdm_batch_mcmc(int nitems, const cdm_batch_item* items, int nburn, int njump
     if (unit->prepare(cdm::kBatchMcmc, nitems, li.data(), host.data() + q * per, &tiles, &lds))
       return fail("batch MCMC: an item does not fit the engine limits");
   }
-  BatchBuffers buf;
-  hipError_t e = hipMalloc(&buf.items, host.size());
-  if (e == hipSuccess) e = hipMalloc((void**)&buf.queues, sizeof(int) * launches.size());
-  if (e == hipSuccess) e = hipMemcpyAsync(buf.items, host.data(), host.size(), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemsetAsync(buf.queues, 0, sizeof(int) * launches.size(), s);
+  std::lock_guard<std::mutex> lock(g_batch_mutex);
+  BatchWorkspace* w = nullptr;
+  hipError_t e = batch_acquire(cdm::kBatchMcmc, host.size(), launches.size(), &w);
+  if (e == hipSuccess) e = batch_order(nitems, items, s);
+  if (e == hipSuccess) e = batch_upload(w, host, s);
   if (e != hipSuccess) return hip_fail(e, "batch MCMC set-up");
   int grid = 0;
   if ((e = unit->grid(cdm::kBatchMcmc, tiles, lds, &grid)) != hipSuccess) return hip_fail(e, "batch MCMC set-up");
@@ -840,19 +1076,35 @@ int cdm_batch_mcmc(int nitems, const cdm_batch_item* items, int nburn, int njump
     size_t q_end = q;
     while (q_end < launches.size() && launches[q_end].phase == phase) ++q_end;
     const size_t q0 = q;
+    // the phase's counters are zeroed by a kernel inside the sequence, so a
+    // replayed graph resets them itself
     auto issue = [&](hipStream_t st) -> hipError_t {
-      for (size_t k = q0; k < q_end; ++k) {
-        const hipError_t el = unit->launch(cdm::kBatchMcmc, (unsigned char*)buf.items + k * per, nitems, tiles,
-                                           buf.queues + k, lds, 0, grid, st);
-        if (el != hipSuccess) return el;
-      }
-      return hipSuccess;
+      hipError_t el = cdm::zero_ints(w->d_queues + q0, (int)(q_end - q0), st);
+      for (size_t k = q0; k < q_end && el == hipSuccess; ++k)
+        el = unit->launch(cdm::kBatchMcmc, (unsigned char*)w->d_items + k * per, nitems, tiles, w->d_queues + k, lds,
+                          0, grid, st);
+      return el;
     };
-    e = (use_graph && q_end - q0 > 1) ? run_as_graph(s, issue) : issue(s);
+    if (use_graph && q_end - q0 > 1) {
+      std::vector<unsigned char> key;
+      const long long shape[8] = {(long long)(size_t)w->d_items, (long long)(size_t)w->d_queues, (long long)per,
+                                  nitems, tiles, (long long)lds, grid, (long long)((q0 << 32) | q_end)};
+      key_add(&key, shape);
+      e = run_graph_cached(w->graphs[phase], &w->capture_stream, &g_batch_stats, key, s, issue);
+    } else {
+      e = issue(s);
+    }
     q = q_end;
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess) e = batch_note(nitems, items, w, s);
   return e == hipSuccess ? 0 : hip_fail(e, "batch MCMC");
+}
+
+int cdm_batch_graph_stats(long long* captures, long long* replays) {
+  std::lock_guard<std::mutex> lock(g_batch_mutex);
+  if (captures) *captures = g_batch_stats.captures;
+  if (replays) *replays = g_batch_stats.replays;
+  return 0;
 }
 
 int cdm_sync(void* stream) {

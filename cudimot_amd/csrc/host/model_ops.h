@@ -65,6 +65,9 @@ struct ModelOps {
   hipError_t (*predicted)(int L, const Geom& g, int nsamples, const float* samples, float* out, hipStream_t s);
   hipError_t (*bic_aic)(int L, int rician, const Geom& g, int nsamples, const float* samples,
                         const float* tau_samples, float* bic, float* aic, hipStream_t s);
+  // 1 when an MCMC launch of group size L keeps the per-measurement cache
+  // rows in LDS (the state buffer then needs no rows after the chains)
+  int (*mcmc_rows_lds)(int L, const Geom& g);
 };
 
 // Voxel-group sizes compiled for every model and method.

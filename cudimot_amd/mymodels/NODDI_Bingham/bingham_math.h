@@ -166,6 +166,34 @@ MACRO void bingham_fan_local(T th, T ph, T psi, T* CFP, T& gx, T& gy, T& gn) {
   gx = CFP[0] * gx_local0 + CFP[1] * gx_local1 + CFP[2] * gx_local2;
 }
 
+// The same frame once per parameter vector, in double: fr[0..2] = f x n,
+// fr[3..5] = the fanning direction f, fr[6..8] = the fibre n; the local
+// gradient of a measurement is then three dot products
+// (bingham_frame_local).
+__device__ inline void bingham_fan_frame(CDM_F64 th, CDM_F64 ph, CDM_F64 psi, CDM_F64* fr) {
+  CDM_F64 st, ct, sp, cp, ss, cs;
+  tsincos(th, &st, &ct);
+  tsincos(ph, &sp, &cp);
+  tsincos(psi, &ss, &cs);
+  const CDM_F64 n0 = st * cp, n1 = st * sp, n2 = ct;
+  const CDM_F64 f0 = -cs * sp - ct * cp * ss, f1 = cp * cs - ct * sp * ss, f2 = st * ss;
+  fr[0] = f1 * n2 - f2 * n1;
+  fr[1] = f2 * n0 - f0 * n2;
+  fr[2] = f0 * n1 - f1 * n0;
+  fr[3] = f0;
+  fr[4] = f1;
+  fr[5] = f2;
+  fr[6] = n0;
+  fr[7] = n1;
+  fr[8] = n2;
+}
+
+MACRO void bingham_frame_local(const CDM_F64* fr, T* CFP, T& gx, T& gy, T& gn) {
+  gx = CFP[0] * (T)fr[0] + CFP[1] * (T)fr[1] + CFP[2] * (T)fr[2];
+  gy = CFP[0] * (T)fr[3] + CFP[1] * (T)fr[4] + CFP[2] * (T)fr[5];
+  gn = CFP[0] * (T)fr[6] + CFP[1] * (T)fr[7] + CFP[2] * (T)fr[8];
+}
+
 // Hindered compartment of one fan from the local gradient (the azimuth
 // ratio and the exponential in their fast forms: both arguments are
 // products of single-precision values already).

@@ -188,13 +188,18 @@ __device__ inline void noddi_watson_sh(CDM_F64 k, CDM_F64* c) {
 // error ~60-fold by I_6 at b = 700 s/mm^2, enough to move the intra-cellular
 // signal by up to ~1e-3 at large kappa; here erf is double, except with
 // CDM_WATSON_REFERENCE_FORMS (the NODDI_Watson_refnum plug-in), which takes
-// the reference's single-precision erf of the float-rounded argument.
+// the reference's single-precision erf of the float-rounded argument.  That
+// value is taken correctly rounded (the double erf rounded to float), one
+// fixed choice: math libraries' erff differ from it by an ulp for a few
+// percent of arguments (glibc 2.35: 3.6% of floats in [0.2, 4]; CUDA's
+// erff allows 2 ulp), and one ulp of erff moves the reference's
+// intra-cellular signal by up to 1.4e-2 (kappa 30) through the recurrence.
 __device__ inline void noddi_lgi(CDM_F64 x, CDM_F64* L) {
   if (x > 0.05) {
     CDM_F64 I[7];
     const CDM_F64 sx = sqrt(x);
 #ifdef CDM_WATSON_REFERENCE_FORMS
-    I[0] = sqrt(3.14159265358979323846) * (CDM_F64)erff((float)sx) / sx;
+    I[0] = sqrt(3.14159265358979323846) * (CDM_F64)(float)erf((CDM_F64)(float)sx) / sx;
 #else
     I[0] = sqrt(3.14159265358979323846) * erf(sx) / sx;
 #endif

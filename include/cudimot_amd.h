@@ -15,7 +15,11 @@
  *   tau      [nvox][nsamp]            MCMC.cu:624
  * `stream` is a hipStream_t (NULL: the null stream); calls are asynchronous
  * on it.  A model handle is not safe for concurrent use from several host
- * threads.
+ * threads.  Its LM work-queue counter and MCMC chain state belong to the
+ * handle: an LM or MCMC call on another stream than the handle's previous
+ * one waits on the device for that call to finish (outside stream capture;
+ * a capturing caller orders its streams itself).  Use one handle per stream
+ * for concurrent fits.
  */
 #ifndef CUDIMOT_AMD_H
 #define CUDIMOT_AMD_H
@@ -98,8 +102,14 @@ int cdm_model_load_tuning(cdm_model* m, const char* path);
 
 /* MCMC jump blocks: split each MCMC phase into launches of at most
  * `jumps_per_launch` jumps (0: one launch per phase); use_graph != 0
- * records the launches of a phase into a HIP graph and replays it. */
+ * records the launches of a phase into a HIP graph, kept on the handle and
+ * replayed by later calls with the same arguments (buffers, sizes, seed,
+ * bounds, launch shape); other arguments record a new graph. */
 int cdm_model_set_mcmc_blocks(cdm_model* m, int jumps_per_launch, int use_graph);
+
+/* Graphs recorded (captures) and replayed from the handle's cache (no
+ * counterpart in the reference; for tests and launch tuning). */
+int cdm_model_graph_stats(const cdm_model* m, long long* captures, long long* replays);
 
 /* GridSearch<T>::run (GridSearch.cu:291-317; kernel :97-248).  Updates the
  * grid parameters of `params` to the combination of least squared error
@@ -232,16 +242,26 @@ typedef struct cdm_batch_item {
  * full length needed. */
 int cdm_batch_units(char* buf, int buflen);
 
-/* Levenberg_Marquardt<T>::run over every item in one launch. */
+/* Levenberg_Marquardt<T>::run over every item in one launch.  The batched
+ * calls are asynchronous on `stream`; their item tables and work-queue
+ * counters live in a per-process workspace (one per method and device,
+ * grown as needed), so a batched call first waits on the host for the
+ * previous call of the same method on that device to finish. */
 int cdm_batch_levmar(int nitems, const cdm_batch_item* items, int max_iters, int marquardt, void* stream);
 
 /* MCMC<T>::run over every item: one launch per jump block (jumps_per_launch
  * jumps, 0: a whole phase), each over all items; use_graph != 0 records a
- * phase's launches into a HIP graph.  Draws are those of cdm_mcmc with the
- * same seed and voxel offsets.  Gaussian likelihood only (rician must be 0). */
+ * phase's launches into a HIP graph, replayed by later calls of the same
+ * shape (the workspace's graphs, cdm_batch_graph_stats).  Draws are those of
+ * cdm_mcmc with the same seed and voxel offsets.  Gaussian likelihood only
+ * (rician must be 0). */
 int cdm_batch_mcmc(int nitems, const cdm_batch_item* items, int nburn, int njumps, int sampleevery,
                    int update_proposal, int rician, unsigned long long seed, int jumps_per_launch, int use_graph,
                    void* stream);
+
+/* Graphs recorded and replayed by cdm_batch_mcmc in this process (no
+ * counterpart in the reference). */
+int cdm_batch_graph_stats(long long* captures, long long* replays);
 
 /* Orientation initialisation from a tensor fit (the reference pipeline's
  * dtifit V1/V2 -> cart2spherical -> initialise_Psi steps: utils/Run_dtifit.sh:36-46,

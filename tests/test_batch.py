@@ -6,7 +6,9 @@ import pytest
 
 from tests.helpers import dev, host
 
-BATCH = [("DiffusionTensor", 300, 4), ("NODDI_Bingham_2Fans", 160, 16), ("Ball_3_Sticks", 250, 4)]
+# (model, voxels, voxel-group size): NODDI_Bingham_2Fans at its tuned 64
+# lanes (the column-wise Jacobian, J^T J in LDS)
+BATCH = [("DiffusionTensor", 300, 4), ("NODDI_Bingham_2Fans", 160, 64), ("Ball_3_Sticks", 250, 4)]
 
 
 def test_default_batch_unit_is_compiled():
@@ -145,3 +147,103 @@ def test_mcmc_jump_blocks_do_not_change_draws(graph):
         out.append((host(p), host(s)))
     np.testing.assert_array_equal(out[0][1], out[1][1])
     np.testing.assert_array_equal(out[0][0], out[1][0])
+
+
+@pytest.mark.gpu
+def test_mcmc_graph_cached_and_replayed():
+    """A blocked MCMC call with use_graph records each phase's launches once;
+    a second call with the same arguments replays the kept graphs (no new
+    capture) and gives bit-identical draws, equal to the unblocked launch;
+    other arguments (a new seed) record new graphs.  Prints the host time
+    of issuing one launch per jump against replaying the recorded graphs."""
+    import time
+    import torch
+    from cudimot_amd import capi
+    v = _volumes(700)[0]
+    m = capi.Model(v["name"])
+    m.set_group(capi.MCMC, v["L"])
+    nb, nj, se = 30, 40, 10
+
+    # the same output buffers in every call: a graph bakes its arguments in
+    p = dev(v["start"])
+    s = torch.empty((v["n"], m.nparams, nj // se), device="cuda")
+    start = dev(v["start"])
+
+    def run(model, seed=5):
+        p.copy_(start)
+        model.mcmc(v["meas"], p, v["cfp"], v["fixp"], s, None, nburn=nb, njumps=nj, sampleevery=se, seed=seed)
+        capi.sync()
+        return host(p), host(s)
+    ref = run(m)
+    m.set_mcmc_blocks(1, use_graph=True)  # one launch per jump
+    first = run(m)
+    assert m.graph_stats() == (2, 0)
+    second = run(m)
+    assert m.graph_stats() == (2, 2)
+    for got in (first, second):
+        np.testing.assert_array_equal(got[0], ref[0])
+        np.testing.assert_array_equal(got[1], ref[1])
+    run(m, seed=6)
+    assert m.graph_stats() == (4, 2)
+    # host cost of submitting 70 launches, against replaying two graphs
+    mu = capi.Model(v["name"])
+    mu.set_group(capi.MCMC, v["L"])
+    mu.set_mcmc_blocks(1, use_graph=False)
+    t_issue, t_replay = [], []
+    for _ in range(5):
+        capi.sync()
+        t0 = time.perf_counter()
+        run(mu)
+        t_issue.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        run(m, seed=6)
+        t_replay.append(time.perf_counter() - t0)
+    print("MCMC, 70 jump launches: issued %.3f ms, replayed from the cached graphs %.3f ms (median of 5)" %
+          (1e3 * np.median(t_issue), 1e3 * np.median(t_replay)))
+
+
+@pytest.mark.gpu
+def test_batch_mcmc_graph_replayed_and_async():
+    """Two identical batched MCMC calls with graphs: the second replays the
+    workspace's graphs, both are bit-identical; the calls return without
+    synchronising the stream (results read after capi.sync)."""
+    import torch
+    from cudimot_amd import capi
+    vols = _volumes(800)
+    models = _models(vols, capi.MCMC)
+    outs = []
+    c0, r0 = capi.batch_graph_stats()
+    for rep in range(2):
+        items = []
+        off = 0
+        for v, m in zip(vols, models):
+            items.append(dict(model=m, meas=v["meas"], params=dev(v["start"]), cfp=v["cfp"], fixp=v["fixp"],
+                              samples=torch.empty((v["n"], m.nparams, 4), device="cuda"), voxel_offset=off))
+            off += v["n"]
+        capi.batch_mcmc(items, nburn=40, njumps=40, sampleevery=10, seed=9, jumps_per_launch=10, use_graph=True)
+        outs.append(items)
+    capi.sync()
+    c1, r1 = capi.batch_graph_stats()
+    assert (c1 - c0, r1 - r0) == (2, 2)
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(host(a["samples"]), host(b["samples"]))
+        np.testing.assert_array_equal(host(a["params"]), host(b["params"]))
+
+
+def test_batch_lm_wide_item_falls_back_like_per_model_launch():
+    """A NODDI_Bingham_2Fans item with 400 measurements at 64 LM lanes needs
+    more LDS than a CU has (its J rows); the batch preparation falls back to
+    16 lanes as the per-model launch does instead of refusing the item.
+    Checked on the host: preparation passes, and only the device work that
+    follows can fail here."""
+    from cudimot_amd import capi
+    m = capi.Model("NODDI_Bingham_2Fans")
+    m.set_group(capi.LEVMAR, 64)
+    dt, d3 = capi.Model("DiffusionTensor"), capi.Model("Ball_3_Sticks")
+    items = [dict(model=m, meas=_Fake((1000, 400)), params=8, cfp=8, fixp=8),
+             dict(model=dt, meas=_Fake((10, 64)), params=8, cfp=8),
+             dict(model=d3, meas=_Fake((10, 280)), params=8, cfp=8)]
+    try:
+        capi.batch_levmar(items)
+    except capi.CdmError as e:
+        assert "does not fit" not in str(e), e

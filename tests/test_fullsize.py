@@ -7,7 +7,8 @@ subsets (the oracle cannot fit 500k voxels in test time):
   * Ball_2_Sticks, 500k voxels, M=280: GridSearch -> LM -> MCMC (1000 burn-in,
     1250 jumps, a sample every 25);
   * DiffusionTensor / NODDI_Bingham_2Fans / Ball_3_Sticks, 400k voxels each:
-    one batched launch per method, LM -> MCMC (500 burn-in, 1250 jumps).
+    one batched launch per method, LM (200 iterations) -> MCMC (500 burn-in,
+    1250 jumps), LM checked against the oracle on a random subset.
 
 Properties: every output finite and inside the bounds; LM never ends above
 its start; GridSearch picks the oracle's combination on every sampled voxel;
@@ -181,9 +182,30 @@ def test_batch_400k_three_models_lm_mcmc():
         items.append(it)
         vols.append((name, cfp, start, meas, fx, L_lm, L_mc, offset))
         offset += n
-    capi.batch_levmar(items, max_iters=20)
+    capi.batch_levmar(items, max_iters=200)
     capi.sync()
     lm_out = [host(it["params"]) for it in items]
+    # LM against the oracle from the same starts on a random subset of each
+    # model: the GPU not in worse minima more often than better (+5%), and
+    # most voxels at the oracle's cost
+    rng = np.random.default_rng(5)
+    for (name, cfp, start, meas, fx, L_lm, L_mc, off), p_lm in zip(vols, lm_out):
+        idx = np.sort(rng.choice(n, 64, replace=False))
+        o = bind.OracleModel(name)
+        sp = {k: np.asarray(v) for k, v in capi.Model(name).spec().items() if k != "init_provided"}
+        meas_s = host(meas)[idx].astype(np.float32).astype(np.float64)
+        cfp32 = cfp.astype(np.float32).astype(np.float64)
+        fx_s = host(fx)[idx] if fx is not None else None
+        p_ref, _, _ = o.levmar(sp, meas_s, start[idx].astype(np.float32).astype(np.float64), cfp32, fx_s,
+                               max_iters=200)
+        cost = lambda q: ((o.signal(q.astype(np.float32).astype(np.float64), cfp32, fx_s) - meas_s) ** 2).sum(1)
+        lr = np.log(cost(p_lm[idx]) / cost(p_ref))  # both after FixConstraintsLM, float-rounded
+        worse, better, same = np.mean(lr > 1e-3), np.mean(lr < -1e-3), np.mean(np.abs(lr) <= 1e-3)
+        print(name, "400k batched LM vs oracle on 64 voxels: worse", worse, "better", better, "same", same)
+        # NODDI_Bingham_2Fans: the oracle against itself after arithmetic-level
+        # perturbations agrees on ~72% of the voxels (test_levmar_2fans_unbiased)
+        assert worse <= better + (0.1 if name == "NODDI_Bingham_2Fans" else 0.05) + 1e-12, (name, worse, better)
+        assert same >= (0.6 if name == "NODDI_Bingham_2Fans" else 0.75), (name, same)
     capi.batch_mcmc(items, nburn=500, njumps=1250, sampleevery=25, seed=8219)
     capi.sync()
     a, k = 200_000, 4096
@@ -197,7 +219,7 @@ def test_batch_400k_three_models_lm_mcmc():
         h.set_group(capi.MCMC, L_mc)
         ps = dev(start[a:a + k])
         fxs = fx[a:a + k].contiguous() if fx is not None else None
-        h.levmar(meas[a:a + k].contiguous(), ps, dev(cfp), fxs, max_iters=20)
+        h.levmar(meas[a:a + k].contiguous(), ps, dev(cfp), fxs, max_iters=200)
         capi.sync()
         np.testing.assert_array_equal(host(ps), p_lm[a:a + k], err_msg=name)
         ss = torch.empty((k, h.nparams, 50), device="cuda")

@@ -4,16 +4,24 @@ oracle (oracle/oracle_engine.h), model by model, on seeded synthetic volumes.
 Tolerances (stated per check):
   * forward signal (FP32 on GPU vs FP64): max |rel| <= 2e-5 of the voxel's S0 scale
   * GridSearch: kept combination index identical for every voxel
-  * Levenberg-Marquardt: final cost within 1e-3 relative for >= 97% of voxels
-    (the rest unbiased), and S0/diffusivity/fraction estimates within 1e-3
-    relative for >= 90%; NODDI_Bingham (flat ridges): within 3% (4% for the
-    parameters) of the oracle's agreement with itself under one-ulp data or
-    1e-12 start changes; multi-stick models (multi-modal): GPU not worse than the oracle
-    more often than better
+  * Levenberg-Marquardt (costs of both sides at the parameters as returned:
+    after FixConstraintsLM, rounded to float): final cost within 1e-3
+    relative for >= 97% of voxels (the rest unbiased), and
+    S0/diffusivity/fraction estimates within 1e-3 relative for >= 90%;
+    NODDI_Bingham (flat ridges): within 3% (4% for the parameters) of the
+    oracle's agreement with itself under one-ulp data or 1e-12 start
+    changes; NODDI_Bingham_2Fans (chaotic): within 5% of that agreement, and
+    no larger excess of worse over better minima (+8%), at 16 and 64 lanes;
+    multi-stick models (multi-modal): GPU not worse than the oracle more
+    often than better (+5%), at 16 and 64 lanes
   * MCMC: draw-for-draw identical samples (<= 1e-4 rel) for >= 95% of voxels
-    over the first jumps; posterior means within 0.35 posterior SD (median
-    over voxels) for long chains
-  * BIC/AIC: within 1e-4 relative
+    over the first jumps, every model; posterior means within 0.35 posterior
+    SD (median over voxels) and SD ratios within 0.8-1.25 (median) for long
+    chains, also at the reference's defaults (1000 burn-in, 1250 jumps, a
+    sample every 25) for NODDI_Bingham, NODDI_Watson and Ball_2_Sticks
+  * BIC/AIC: within 1e-4 relative, every model
+  * NODDI_Watson_refnum: the reference plug-in's GridSearch argmin except
+    exact ties of its FP64 costs
 """
 import numpy as np
 import pytest
@@ -149,12 +157,24 @@ def test_watson_refnum_gridsearch_matches_reference_plugin(torch_cuda):
         capi.sync()
         kept[model] = k.cpu().numpy()
     spec = _spec_arrays(capi.Model("NODDI_Watson_refnum").spec())
-    _, kept_ref = ref.gridsearch(spec, noisy.astype(np.float32).astype(np.float64), p0.astype(np.float32).astype(
-        np.float64), cfp.astype(np.float32).astype(np.float64), fx, np.asarray(gp), np.asarray(grid))
+    noisy32, cfp32 = noisy.astype(np.float32).astype(np.float64), cfp.astype(np.float32).astype(np.float64)
+    _, kept_ref = ref.gridsearch(spec, noisy32, p0.astype(np.float32).astype(np.float64), cfp32, fx, np.asarray(gp),
+                                 np.asarray(grid))
     moved = np.mean(kept["NODDI_Watson"] != kept_ref)
+    mism = np.nonzero(kept["NODDI_Watson_refnum"] != kept_ref)[0]
     print("refnum vs reference plug-in: %d mismatches; NODDI_Watson moves %.2f%% of the argmins" %
-          (np.sum(kept["NODDI_Watson_refnum"] != kept_ref), 100 * moved))
-    np.testing.assert_array_equal(kept["NODDI_Watson_refnum"], kept_ref)
+          (mism.size, 100 * moved))
+
+    def cost(v, k):
+        q = p0[v].copy()
+        q[np.asarray(gp)] = np.asarray(grid)[k]
+        return ((ref.signal(q[None], cfp32, fx[v:v + 1])[0] - noisy32[v]) ** 2).sum()
+    # the reference plug-in's closed forms carry ~1e-9 of rounding noise near
+    # kappa = 1 (c5, c6 cancel to 1e-18 of their terms), so only a tie of the
+    # reference's own FP64 costs may flip
+    for v in mism:
+        a, b = cost(v, kept["NODDI_Watson_refnum"][v]), cost(v, kept_ref[v])
+        assert abs(a - b) <= 1e-7 * b, (v, a, b)
 
 
 def _clip_noddi(model, p):
@@ -210,12 +230,22 @@ def _near_truth(model, t, rng):
 _ORACLE_LM = {}
 
 
+def _cost_at(o, p, cfp32, fx, noisy32):
+    """FP64 cost at parameters as the C-ABI hands them back: after the
+    plug-in's FixConstraintsLM and rounded to float (the oracle's own
+    returned cost is the one before FixConstraintsLM, which for the NODDI
+    models can move beta after the fit)."""
+    q = np.asarray(p, dtype=np.float64).astype(np.float32).astype(np.float64)
+    return ((o.signal(q, cfp32, fx) - noisy32) ** 2).sum(1)
+
+
 def _oracle_levmar(model, start, n):
     """Volume, start and the oracle's LM result, once per (model, start, n)
-    for all group sizes.  For NODDI_Bingham also the oracle's own spread,
-    the lower of two: LM again on the data with every value moved by about
-    one FP32 ulp, and from starts moved by 1e-12 relative (arithmetic-level
-    noise)."""
+    for all group sizes.  For the Bingham-NODDI models also the oracle's own
+    spread over two arithmetic-level perturbations: LM again on the data with
+    every value moved by about one FP32 ulp, and from starts moved by 1e-12
+    relative: the lowest agreement fractions and the largest excess of
+    worse over better minima of the two."""
     key = (model, start, n)
     if key not in _ORACLE_LM:
         from cudimot_amd import capi
@@ -226,19 +256,19 @@ def _oracle_levmar(model, start, n):
         noisy32 = noisy.astype(np.float32).astype(np.float64)
         cfp32 = cfp.astype(np.float32).astype(np.float64)
         p0_32 = p0.astype(np.float32).astype(np.float64)
-        p_ref, _, c_ref = o.levmar(_spec_arrays(spec), noisy32, p0_32, cfp32, fx, max_iters=200)
+        p_ref, _, _ = o.levmar(_spec_arrays(spec), noisy32, p0_32, cfp32, fx, max_iters=200)
+        c_ref = _cost_at(o, p_ref, cfp32, fx, noisy32)
         spread = None
-        if model == "NODDI_Bingham":
+        if model in ("NODDI_Bingham", "NODDI_Bingham_2Fans"):
             ulp = 1.2e-7 * np.random.default_rng(5).choice([-1.0, 1.0], size=noisy32.shape)
             pert = (noisy32 * (1 + ulp)).astype(np.float32).astype(np.float64)
             p_s, _, _ = o.levmar(_spec_arrays(spec), pert, p0_32, cfp32, fx, max_iters=200)
-            c_s = ((o.signal(p_s, cfp32, fx) - noisy32) ** 2).sum(1)
-            a = _agreement(model, p_s, p_ref, c_s, c_ref)
+            a = _agreement(model, p_s, p_ref, _cost_at(o, p_s, cfp32, fx, noisy32), c_ref)
             p0_s = p0_32 * (1 + 1e-12 * np.random.default_rng(6).choice([-1.0, 1.0], size=p0_32.shape))
             p_s, _, _ = o.levmar(_spec_arrays(spec), noisy32, p0_s, cfp32, fx, max_iters=200)
-            c_s = ((o.signal(p_s, cfp32, fx) - noisy32) ** 2).sum(1)
-            b = _agreement(model, p_s, p_ref, c_s, c_ref)
-            spread = (min(a[0], b[0]), max(a[1], b[1]), max(a[2], b[2]), min(a[3], b[3]))
+            b = _agreement(model, p_s, p_ref, _cost_at(o, p_s, cfp32, fx, noisy32), c_ref)
+            spread = dict(cost=min(a[0], b[0]), worse=max(a[1], b[1]), better=max(a[2], b[2]), params=min(a[3], b[3]),
+                          excess=max(a[1] - a[2], b[1] - b[2]))
         _ORACLE_LM[key] = (o, cfp, fx, noisy, p0, noisy32, cfp32, p_ref, c_ref, spread)
     return _ORACLE_LM[key]
 
@@ -248,7 +278,7 @@ def _run_levmar(torch_cuda, model, lanes, start, n=None):
     m = capi.Model(model)
     m.set_group(capi.LEVMAR, lanes)
     n = n or {"Ball_2_Sticks": 256, "Ball_3_Sticks": 256, "NODDI_Watson": 256, "NODDI_Bingham": 1024,
-              "NODDI_Bingham_2Fans": 128}.get(model, 1024)
+              "NODDI_Bingham_2Fans": 256}.get(model, 1024)
     o, cfp, fx, noisy, p0, noisy32, cfp32, p_ref, c_ref, spread = _oracle_levmar(model, start, n)
     dp = H.dev(p0)
     cost = torch_cuda.zeros((n,), dtype=torch_cuda.float32, device="cuda")
@@ -256,12 +286,13 @@ def _run_levmar(torch_cuda, model, lanes, start, n=None):
     m.levmar(H.dev(noisy), dp, H.dev(cfp), H.dev(fx), max_iters=200, iters=iters, cost=cost)
     capi.sync()
     p = H.host(dp)
-    c_at = ((o.signal(p, cfp32, fx) - noisy32) ** 2).sum(1)  # cost at the GPU solution, FP64
+    c_at = _cost_at(o, p, cfp32, fx, noisy32)  # cost at the GPU solution, FP64
     return p, p_ref, c_at, c_ref, H.host(cost), spread
 
 
 LM_IDX = {"Ball_1_Stick": [0, 1, 2], "Ball_2_Sticks": [0, 1], "Ball_3_Sticks": [0, 1],
-          "DiffusionTensor": [0, 1, 4, 6], "toymodel": [0, 1], "NODDI_Watson": [1, 2], "NODDI_Bingham": [1, 2]}
+          "DiffusionTensor": [0, 1, 4, 6], "toymodel": [0, 1], "NODDI_Watson": [1, 2], "NODDI_Bingham": [1, 2],
+          "NODDI_Bingham_2Fans": [0, 1, 2, 8]}
 
 
 def _agreement(model, p, p_ref, c_at, c_ref):
@@ -294,7 +325,7 @@ def test_levmar_parity(torch_cuda, model, lanes):
           "oracle self-spread", spread)
     need_cost, need_p = 0.97, 0.90
     if spread is not None:
-        need_cost, need_p = spread[0] - 0.03, spread[3] - 0.04
+        need_cost, need_p = spread["cost"] - 0.03, spread["params"] - 0.04
     assert frac_cost >= need_cost, (model, lanes, frac_cost, need_cost)
     assert worse <= better + 0.03, (worse, better)
     assert np.median(np.abs(c - c_at) / np.maximum(c_at, 1e-30)) < 1e-4
@@ -323,19 +354,26 @@ def test_levmar_multistick_unbiased(torch_cuda, model, start, lanes):
 
 @pytest.mark.parametrize("lanes", [16, 64])
 def test_levmar_2fans_unbiased(torch_cuda, lanes):
-    """NODDI_Bingham_2Fans from starts near the truth (the tuned launch is
-    64 lanes, whose LDS normal sums take two rounds of the wave for its 119
-    entries; 16 lanes keep per-lane accumulators).  The fit is chaotic (two
-    fans, flat fanning angles), so the GPU must not end in worse minima more
-    often than better ones and most voxels must agree."""
-    p, p_ref, c_at, c_ref, c, _ = _run_levmar(torch_cuda, "NODDI_Bingham_2Fans", lanes, "near")
+    """NODDI_Bingham_2Fans from starts near the truth, 256 voxels (the tuned
+    launch is 64 lanes: the Jacobian column by column, J^T J summed and
+    factorised in LDS, two rounds of the wave for its 119 sums; 16 lanes keep
+    per-lane accumulators and recompute probe caches per measurement).  The
+    fit is chaotic (two fans, flat fanning angles): the oracle itself, after a
+    one-ulp data change or a 1e-12 start change, ends at another cost for
+    ~30% of the voxels (measured at 256 voxels: same 0.715, worse 0.168,
+    better 0.117).  The GPU must agree with the oracle as well as the oracle
+    agrees with itself (within 5%), with no larger excess of worse over
+    better minima than the oracle's own, within 8% (two standard errors of
+    the difference of two such excesses over 256 voxels)."""
+    p, p_ref, c_at, c_ref, c, spread = _run_levmar(torch_cuda, "NODDI_Bingham_2Fans", lanes, "near")
     lr = np.log(c_at / c_ref)
     worse, better, same = np.mean(lr > 1e-3), np.mean(lr < -1e-3), np.mean(np.abs(lr) <= 1e-3)
-    print("2Fans", lanes, "GPU worse", worse, "better", better, "same", same, "median", np.median(lr))
+    print("2Fans", lanes, "GPU worse", worse, "better", better, "same", same, "median", np.median(lr),
+          "oracle self-spread", spread)
     assert np.all(np.isfinite(p))
     assert abs(np.median(lr)) < 1e-3, np.median(lr)
-    assert worse <= better + 0.05, (worse, better)
-    assert same >= 0.5, same
+    assert same >= spread["cost"] - 0.05, (same, spread)
+    assert worse - better <= max(spread["excess"], 0.0) + 0.08, (worse, better, spread)
     assert np.median(np.abs(c - c_at) / np.maximum(c_at, 1e-30)) < 1e-4
 
 
