@@ -253,7 +253,7 @@ def build(models=None, extra_dirs=(), force=False, jobs=None, batches=None, out_
             f.write(text)
     host_deps = [os.path.join(CSRC, "host"), os.path.join(PKG, "..", "include", "cudimot_amd.h")]
     jobs_list.append((reg, os.path.join(build_dir, "registry.o"), host_deps))
-    for f in ("capi.cpp", "spec.cpp", "fill.hip"):
+    for f in ("capi.cpp", "spec.cpp", "noalloc.hip", "fill.hip"):
         jobs_list.append((os.path.join(CSRC, "host", f), os.path.join(build_dir, f.split(".")[0] + ".o"), host_deps))
     jobs_list.append((os.path.join(CSRC, "mlp", "mlp.hip"), os.path.join(build_dir, "mlp.o"),
                       host_deps + [os.path.join(CSRC, "mlp"), os.path.join(CSRC, "engine", "philox.h")]))

@@ -59,6 +59,7 @@ def lib():
             "cdm_gridsearch": (i, [vp, i, i, vp, vp, vp, vp, vp, vp]),
             "cdm_levmar": (i, [vp, i, i, vp, vp, vp, vp, i, i, vp, vp, vp]),
             "cdm_mcmc": (i, [vp, i, i, vp, vp, vp, vp, i, i, i, i, i, u64, i64, vp, vp, vp]),
+            "cdm_mcmc_noalloc": (i, [vp, i, i, vp, vp, vp, vp, i, i, i, i, i, u64, i64, vp, vp, vp]),
             "cdm_predicted_signal": (i, [vp, i, i, i, vp, vp, vp, vp, vp]),
             "cdm_bic_aic": (i, [vp, i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp]),
             "cdm_rician_clean": (i, [i, i, vp]),
@@ -223,6 +224,15 @@ class Model:
         check(lib().cdm_mcmc(self.h, nvox, nmeas, ptr(meas), ptr(params), ptr(cfp), ptr(fixp), int(nburn),
                              int(njumps), int(sampleevery), 1 if update_proposal else 0, 1 if rician else 0,
                              int(seed), int(voxel_offset), ptr(samples), ptr(tau_samples), _stream(stream)))
+
+    def mcmc_noalloc(self, meas, params, cfp, fixp, samples, tau_samples=None, nburn=1000, njumps=1250,
+                     sampleevery=25, update_proposal=True, rician=False, seed=8219, voxel_offset=0, stream=None):
+        """cdm_mcmc_noalloc: MCMC with the chain state in the library's static
+        device scratch (no device allocation; a bounded voxel count)."""
+        nvox, nmeas = meas.shape
+        check(lib().cdm_mcmc_noalloc(self.h, nvox, nmeas, ptr(meas), ptr(params), ptr(cfp), ptr(fixp), int(nburn),
+                                     int(njumps), int(sampleevery), 1 if update_proposal else 0, 1 if rician else 0,
+                                     int(seed), int(voxel_offset), ptr(samples), ptr(tau_samples), _stream(stream)))
 
     def predicted_signal(self, samples, cfp, fixp, out, stream=None):
         nvox, _, ns = samples.shape

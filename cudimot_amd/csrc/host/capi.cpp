@@ -18,6 +18,7 @@
 #include "../../../include/cudimot_amd.h"
 #include "fill.h"
 #include "model_ops.h"
+#include "noalloc.h"
 #include "spec.h"
 
 namespace {
@@ -620,7 +621,7 @@ int cdm_levmar(cdm_model* m, int nvox, int nmeas, const float* meas, float* para
 
 namespace {
 
-// Argument checks of cdm_mcmc (no device work).
+// Argument checks shared by cdm_mcmc and cdm_mcmc_noalloc (no device work).
 int check_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, const float* cfp, const float* fixp, int njumps,
                int sampleevery, int rician, const float* samples, const float* tau_samples) {
   int r = check_geom(m, nvox, nmeas, meas, cfp, fixp, true);
@@ -727,6 +728,28 @@ int cdm_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params
   if (r) return r;
   e = note_last(m, s);
   return e == hipSuccess ? 0 : hip_fail(e, "MCMC stream order");
+}
+
+int cdm_mcmc_noalloc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params, const float* cfp,
+                     const float* fixp, int nburn, int njumps, int sampleevery, int update_proposal, int rician,
+                     unsigned long long seed, long long voxel_offset, float* samples, float* tau_samples,
+                     void* stream) {
+  int r = check_mcmc(m, nvox, nmeas, meas, cfp, fixp, njumps, sampleevery, rician, samples, tau_samples);
+  if (r || nvox == 0) return r;
+  const size_t need = mcmc_state_bytes(m, nvox, nmeas, cfp, fixp);
+  if (need > cdm::kNoAllocScratchBytes) {
+    const size_t per = (need + (size_t)nvox - 1) / (size_t)nvox;
+    return fail("cdm_mcmc_noalloc: " + std::to_string(nvox) + " voxels x " + std::to_string(nmeas) +
+                " measurements need " + std::to_string(need) + " bytes of chain state, more than the " +
+                std::to_string(cdm::kNoAllocScratchBytes) + "-byte allocation-free scratch (at most about " +
+                std::to_string(cdm::kNoAllocScratchBytes / per) +
+                " voxels per call for this model; split the volume and pass each part's voxel_offset)");
+  }
+  void* scratch = nullptr;
+  const hipError_t e = cdm::noalloc_scratch(&scratch);
+  if (e != hipSuccess) return hip_fail(e, "allocation-free scratch lookup");
+  return run_mcmc(m, nvox, nmeas, meas, params, cfp, fixp, nburn, njumps, sampleevery, update_proposal, rician, seed,
+                  voxel_offset, samples, tau_samples, scratch, false, (hipStream_t)stream);
 }
 
 int cdm_predicted_signal(cdm_model* m, int nvox, int nmeas, int nsamples, const float* samples, const float* cfp,

@@ -135,6 +135,26 @@ int cdm_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params
              const float* fixp, int nburn, int njumps, int sampleevery, int update_proposal, int rician,
              unsigned long long seed, long long voxel_offset, float* samples, float* tau_samples, void* stream);
 
+/* cdm_mcmc without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and draws as cdm_mcmc (bit-identical samples and
+ * parameters); the chain state (and, for models whose per-measurement cache
+ * rows do not fit in LDS, those 2 x nmeas rows per voxel) lives in a
+ * library-owned static device scratch of 64 MiB per GPU, held until the
+ * process ends, instead of the handle's allocation.  A volume needing more
+ * (NODDI_Bingham: 256 bytes of chain state per voxel, 262,144 voxels) is
+ * refused with an error: split it and pass each part's voxel_offset, which
+ * keeps the random streams.  Every call
+ * on a GPU shares that GPU's scratch: calls that overlap on one GPU (other
+ * streams, other handles, other host threads) race.  No jump-block graphs
+ * (cdm_model_set_mcmc_blocks' use_graph is ignored; jump blocks still
+ * apply). */
+int cdm_mcmc_noalloc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params, const float* cfp,
+                     const float* fixp, int nburn, int njumps, int sampleevery, int update_proposal, int rician,
+                     unsigned long long seed, long long voxel_offset, float* samples, float* tau_samples,
+                     void* stream);
+
 /* getPredictedSignal<T>::run (getPredictedSignal.cu:98-118): signal of the
  * mean sample, out [nvox][nmeas]. */
 int cdm_predicted_signal(cdm_model* m, int nvox, int nmeas, int nsamples, const float* samples, const float* cfp,
