@@ -114,9 +114,9 @@ def batch_unit_source(name, model_dirs):
                 "template <int L>",
                 "__device__ __noinline__ void batch_lm(const void* a, int marquardt, int blk) {",
                 "  cdm_levmar_body<L>(*static_cast<const cdm_lm_args*>(a), marquardt, blk);", "}",
-                "template <int L>",
+                "template <int L, bool RICIAN>",
                 "__device__ __noinline__ void batch_mcmc(const void* a, int blk) {",
-                "  cdm_mcmc_body<L, false>(*static_cast<const cdm_mcmc_args*>(a), blk);", "}",
+                "  cdm_mcmc_body<L, RICIAN>(*static_cast<const cdm_mcmc_args*>(a), blk);", "}",
                 "}  // namespace", ""]
     ns = lambda k: "cdm_batch_%s_m%d" % (name, k)
     sizes = ", ".join("sizeof(%s::cdm_lm_args), sizeof(%s::cdm_mcmc_args)" % (ns(k), ns(k))
@@ -136,9 +136,15 @@ def batch_unit_source(name, model_dirs):
             "    const DevItem* d = &items[it];", "    const int blk = tile - d->first_tile;", "    switch (d->kase) {"]
     for k in range(len(model_dirs)):
         for li, l in enumerate(L):
-            out.append("      case %d: if constexpr (MODE == 0) %s::batch_lm<%d>(d->args, marquardt, blk); "
-                       "else %s::batch_mcmc<%d>(d->args, blk); break;" % (4 * k + li, ns(k), l, ns(k), l))
-    out += ["      default: break;", "    }", "  }", "}", ""]
+            out.append("      case %d: if constexpr (MODE == cdm::kBatchLevMar) %s::batch_lm<%d>(d->args, marquardt, blk); "
+                       "else %s::batch_mcmc<%d, MODE == cdm::kBatchMcmcRician>(d->args, blk); break;" %
+                       (4 * k + li, ns(k), l, ns(k), l))
+    out += ["      default: break;", "    }", "  }", "}", "",
+            "// one persistent kernel per mode: LM, MCMC, MCMC with the Rician likelihood",
+            "static auto kernel_of(int mode) {",
+            "  return mode == cdm::kBatchLevMar ? batch_kernel<cdm::kBatchLevMar>",
+            "         : mode == cdm::kBatchMcmc  ? batch_kernel<cdm::kBatchMcmc>",
+            "                                    : batch_kernel<cdm::kBatchMcmcRician>;", "}", ""]
     out += ["static int prepare(int mode, int nitems, const cdm::BatchLaunchItem* in, void* host_out, int* total_tiles,",
             "                   size_t* lds_bytes) {",
             "  DevItem* out = static_cast<DevItem*>(host_out);", "  int tiles = 0;", "  size_t lds = 0;",
@@ -151,7 +157,7 @@ def batch_unit_source(name, model_dirs):
                 "        if (b.g.cfp_ts > %s::cdm_cfp_max || b.g.fixp_ts > %s::cdm_fixp_max) return -1;" % (ns(k), ns(k)),
                 "        // LM groups of 64 lanes past the LDS fall back to 16, as the per-model launch",
                 "        if (mode == cdm::kBatchLevMar) L = %s::op_lm_lanes(b.g, L);" % ns(k),
-                "        need = mode == cdm::kBatchMcmc ? %s::op_lds_mcmc(b.g, L) : %s::op_lds_lm(b.g, L);" % (ns(k), ns(k)),
+                "        need = mode != cdm::kBatchLevMar ? %s::op_lds_mcmc(b.g, L) : %s::op_lds_lm(b.g, L);" % (ns(k), ns(k)),
                 "        if (need > %s::cdm_lds_max) return -1;" % ns(k),
                 "        if (mode == cdm::kBatchLevMar) {",
                 "          const auto a = %s::op_lm_args(b.g, b.params, b.max_iters, b.c, b.iters, b.cost);" % ns(k),
@@ -166,7 +172,7 @@ def batch_unit_source(name, model_dirs):
             "    lds = std::max(lds, need);", "  }",
             "  *total_tiles = tiles;", "  *lds_bytes = lds;", "  return 0;", "}", "",
             "static hipError_t grid(int mode, int total_tiles, size_t lds, int* out) {",
-            "  auto kern = mode == cdm::kBatchLevMar ? batch_kernel<0> : batch_kernel<1>;",
+            "  auto kern = kernel_of(mode);",
             "  int dev = 0, cus = 0, per_cu = 0;", "  hipError_t e = hipGetDevice(&dev);",
             "  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);",
             "  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, lds);",
@@ -176,7 +182,7 @@ def batch_unit_source(name, model_dirs):
             "static hipError_t launch(int mode, const void* dev_items, int nitems, int total_tiles, int* queue,",
             "                         size_t lds, int marquardt, int grid, hipStream_t s) {",
             "  if (total_tiles <= 0) return hipSuccess;",
-            "  auto kern = mode == cdm::kBatchLevMar ? batch_kernel<0> : batch_kernel<1>;",
+            "  auto kern = kernel_of(mode);",
             "  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, static_cast<const DevItem*>(dev_items), nitems,",
             "                     total_tiles, queue, marquardt);",
             "  return hipGetLastError();", "}", "",

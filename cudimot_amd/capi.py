@@ -275,13 +275,13 @@ def batch_units():
 
 def _batch_items(items):
     """items: dicts with model (Model), meas, params, cfp, fixp (optional),
-    iters, cost, samples (optional) and voxel_offset (optional)."""
+    iters, cost, samples, tau_samples (optional) and voxel_offset (optional)."""
     arr = (BatchItem * len(items))()
     for k, it in enumerate(items):
         meas = it["meas"]
         arr[k] = BatchItem(it["model"].h.value, meas.shape[0], meas.shape[1], ptr(meas), ptr(it["params"]),
                            ptr(it["cfp"]), ptr(it.get("fixp")), ptr(it.get("iters")), ptr(it.get("cost")),
-                           ptr(it.get("samples")), None, int(it.get("voxel_offset", 0)))
+                           ptr(it.get("samples")), ptr(it.get("tau_samples")), int(it.get("voxel_offset", 0)))
     return arr
 
 
@@ -292,11 +292,11 @@ def batch_levmar(items, max_iters=200, marquardt=True, stream=None):
 
 
 def batch_mcmc(items, nburn=1000, njumps=1250, sampleevery=25, update_proposal=True, seed=8219, jumps_per_launch=0,
-               use_graph=False, stream=None):
+               use_graph=False, rician=False, stream=None):
     arr = _batch_items(items)
     check(lib().cdm_batch_mcmc(len(items), ctypes.cast(arr, ctypes.c_void_p), int(nburn), int(njumps),
-                               int(sampleevery), 1 if update_proposal else 0, 0, int(seed), int(jumps_per_launch),
-                               1 if use_graph else 0, _stream(stream)))
+                               int(sampleevery), 1 if update_proposal else 0, 1 if rician else 0, int(seed),
+                               int(jumps_per_launch), 1 if use_graph else 0, _stream(stream)))
 
 
 def tensor_frame(dti_params, out, first=1, stream=None):

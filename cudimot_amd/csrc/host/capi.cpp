@@ -646,14 +646,14 @@ size_t mcmc_state_bytes(const cdm_model* m, int nvox, int nmeas, const float* cf
   return mvc_offset(m, nvox) + m->ops->mvc_bytes_per_meas * 2 * (size_t)nmeas * (size_t)nvox;
 }
 
-// Burn-in then recording, each phase as one launch or as jump blocks
-// (cdm_model_set_mcmc_blocks), over the chain state at `state` (nvox voxels,
-// their cache rows after it).  allow_graph: record a blocked phase into a
-// HIP graph when the handle asks for it.
+// Burn-in then recording, each phase as one launch or as blocks of
+// jumps_per_launch jumps (0: whole phases), over the chain state at `state`
+// (nvox voxels, their cache rows after it).  use_graph: record a blocked
+// phase into a HIP graph kept by the handle and replayed by later calls.
 int run_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params, const float* cfp, const float* fixp,
              int nburn, int njumps, int sampleevery, int update_proposal, int rician, unsigned long long seed,
-             long long voxel_offset, float* samples, float* tau_samples, void* state, bool allow_graph,
-             hipStream_t s) {
+             long long voxel_offset, float* samples, float* tau_samples, void* state, int jumps_per_launch,
+             bool use_graph, hipStream_t s) {
   const int nb = nburn < 1 ? 1 : nburn;  // MCMC.cu:702-703
   const int nsamples = njumps / sampleevery;
   const cdm::Geom g = make_geom(m, nvox, nmeas, meas, cfp, fixp);
@@ -675,7 +675,7 @@ int run_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params
   ml.voxel_offset = voxel_offset;
   for (int phase = 0; phase < 2; ++phase) {
     const int len = phase == 0 ? nb : njumps;
-    const int blk = m->mcmc_block > 0 ? m->mcmc_block : len;
+    const int blk = jumps_per_launch > 0 ? jumps_per_launch : len;
     ml.recording = phase;
     ml.update_prop = phase == 0 ? 1 : (update_proposal ? 1 : 0);
     ml.phase_len = len;
@@ -690,7 +690,7 @@ int run_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params
       return hipSuccess;
     };
     hipError_t e;
-    if (allow_graph && m->mcmc_graph && len > blk) {
+    if (use_graph && len > blk) {
       // the phase's graph is replayed while every argument its launches
       // bake in is the same (the jump-block offsets follow from len, blk)
       std::vector<unsigned char> key;
@@ -710,6 +710,23 @@ int run_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params
   return 0;
 }
 
+// MCMC over the handle's own chain state, ordered after the handle's
+// previous call (arguments already checked).
+int mcmc_call(cdm_model* m, int nvox, int nmeas, const float* meas, float* params, const float* cfp, const float* fixp,
+              int nburn, int njumps, int sampleevery, int update_proposal, int rician, unsigned long long seed,
+              long long voxel_offset, float* samples, float* tau_samples, int jumps_per_launch, bool use_graph,
+              hipStream_t s) {
+  int r = ensure_state(m, mcmc_state_bytes(m, nvox, nmeas, cfp, fixp));
+  if (r) return r;
+  hipError_t e = order_after_last(m, s);
+  if (e != hipSuccess) return hip_fail(e, "MCMC stream order");
+  r = run_mcmc(m, nvox, nmeas, meas, params, cfp, fixp, nburn, njumps, sampleevery, update_proposal, rician, seed,
+               voxel_offset, samples, tau_samples, m->d_state, jumps_per_launch, use_graph, s);
+  if (r) return r;
+  e = note_last(m, s);
+  return e == hipSuccess ? 0 : hip_fail(e, "MCMC stream order");
+}
+
 }  // namespace
 
 extern "C" {
@@ -719,15 +736,8 @@ int cdm_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params
              unsigned long long seed, long long voxel_offset, float* samples, float* tau_samples, void* stream) {
   int r = check_mcmc(m, nvox, nmeas, meas, cfp, fixp, njumps, sampleevery, rician, samples, tau_samples);
   if (r || nvox == 0) return r;
-  if ((r = ensure_state(m, mcmc_state_bytes(m, nvox, nmeas, cfp, fixp)))) return r;
-  hipStream_t s = (hipStream_t)stream;
-  hipError_t e = order_after_last(m, s);
-  if (e != hipSuccess) return hip_fail(e, "MCMC stream order");
-  r = run_mcmc(m, nvox, nmeas, meas, params, cfp, fixp, nburn, njumps, sampleevery, update_proposal, rician, seed,
-               voxel_offset, samples, tau_samples, m->d_state, true, s);
-  if (r) return r;
-  e = note_last(m, s);
-  return e == hipSuccess ? 0 : hip_fail(e, "MCMC stream order");
+  return mcmc_call(m, nvox, nmeas, meas, params, cfp, fixp, nburn, njumps, sampleevery, update_proposal, rician, seed,
+                   voxel_offset, samples, tau_samples, m->mcmc_block, m->mcmc_graph != 0, (hipStream_t)stream);
 }
 
 int cdm_mcmc_noalloc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params, const float* cfp,
@@ -749,7 +759,7 @@ int cdm_mcmc_noalloc(cdm_model* m, int nvox, int nmeas, const float* meas, float
   const hipError_t e = cdm::noalloc_scratch(&scratch);
   if (e != hipSuccess) return hip_fail(e, "allocation-free scratch lookup");
   return run_mcmc(m, nvox, nmeas, meas, params, cfp, fixp, nburn, njumps, sampleevery, update_proposal, rician, seed,
-                  voxel_offset, samples, tau_samples, scratch, false, (hipStream_t)stream);
+                  voxel_offset, samples, tau_samples, scratch, m->mcmc_block, false, (hipStream_t)stream);
 }
 
 int cdm_predicted_signal(cdm_model* m, int nvox, int nmeas, int nsamples, const float* samples, const float* cfp,
@@ -867,6 +877,23 @@ int check_batch(int nitems, const cdm_batch_item* items, bool mcmc, const cdm::B
   return 0;
 }
 
+// An item whose voxels alone fill the GPU several times over (blocks of its
+// tuned voxel group >= 8 per CU) runs through the per-model kernels at its
+// own tuned launch shape, voxel group and register budget, with the same
+// draws and results; the persistent kernel, whose register allocation is the
+// largest of its models' and sets one occupancy for all of them, takes the
+// smaller items, which it packs together to fill the GPU.
+bool fills_gpu(const cdm_model* m, int method, int nmeas, int nvox) {
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) {
+    (void)hipGetLastError();
+    return false;
+  }
+  const long long per_block = 256 / group_for(m, method, nmeas, nvox);
+  return ((long long)nvox + per_block - 1) / per_block >= 8LL * ncu;
+}
+
 // Items ordered by estimated work per voxel (heaviest first), so the queue
 // hands out the long tiles early and the short ones fill the tail.  The
 // estimate is the tuning table's time for the model and method when present,
@@ -977,7 +1004,10 @@ hipError_t batch_note(int nitems, const cdm_batch_item* items, BatchWorkspace* w
 
 extern "C" {
 
-int cdm_batch_levmar(int nitems, const cdm_batch_item* items, int max_iters, int marquardt, void* stream) {
+// One persistent launch per method over all the items (the work queue hands
+// out tiles of every model).
+static int batch_levmar_persistent(int nitems, const cdm_batch_item* items, int max_iters, int marquardt,
+                                   void* stream) {
   const cdm::BatchOps* unit = nullptr;
   std::vector<int> index;
   long long total = 0;
@@ -1017,22 +1047,24 @@ int cdm_batch_levmar(int nitems, const cdm_batch_item* items, int max_iters, int
   return e == hipSuccess ? 0 : hip_fail(e, "batch LM");
 }
 
-int cdm_batch_mcmc(int nitems, const cdm_batch_item* items, int nburn, int njumps, int sampleevery,
-                   int update_proposal, int rician, unsigned long long seed, int jumps_per_launch, int use_graph,
-                   void* stream) {
+static int batch_mcmc_persistent(int nitems, const cdm_batch_item* items, int nburn, int njumps, int sampleevery,
+                                 int update_proposal, int rician, unsigned long long seed, int jumps_per_launch,
+                                 int use_graph, void* stream) {
   const cdm::BatchOps* unit = nullptr;
   std::vector<int> index;
   long long total = 0;
   int r = check_batch(nitems, items, true, &unit, &index, &total);
   if (r) return r;
-  if (rician) return fail("batched MCMC is compiled with the Gaussian likelihood only (run --rician per model)");
   if (sampleevery <= 0) return fail("--sampleevery must be > 0");
   if (njumps < sampleevery || njumps % sampleevery != 0)
     return fail("--nj must be a positive multiple of --se");
   if (jumps_per_launch < 0) return fail("jumps per launch must be >= 0");
   const int nb = nburn < 1 ? 1 : nburn;  // MCMC.cu:702-703
   const int nsamples = njumps / sampleevery;
+  const int mode = rician ? cdm::kBatchMcmcRician : cdm::kBatchMcmc;
   hipStream_t s = (hipStream_t)stream;
+  for (int i = 0; i < nitems; ++i)
+    if (rician && !items[i].tau_samples) return fail("batch item " + std::to_string(i) + ": --rician needs tau_samples");
   for (int i = 0; i < nitems; ++i) {
     const cdm_model* im = items[i].model;
     const size_t bytes = mvc_offset(im, items[i].nvox) +
@@ -1083,7 +1115,7 @@ int cdm_batch_mcmc(int nitems, const cdm_batch_item* items, int nburn, int njump
       ml.begin = L.begin;
       ml.count = L.count;
     }
-    if (unit->prepare(cdm::kBatchMcmc, nitems, li.data(), host.data() + q * per, &tiles, &lds))
+    if (unit->prepare(mode, nitems, li.data(), host.data() + q * per, &tiles, &lds))
       return fail("batch MCMC: an item does not fit the engine limits");
   }
   std::lock_guard<std::mutex> lock(g_batch_mutex);
@@ -1093,7 +1125,7 @@ int cdm_batch_mcmc(int nitems, const cdm_batch_item* items, int nburn, int njump
   if (e == hipSuccess) e = batch_upload(w, host, s);
   if (e != hipSuccess) return hip_fail(e, "batch MCMC set-up");
   int grid = 0;
-  if ((e = unit->grid(cdm::kBatchMcmc, tiles, lds, &grid)) != hipSuccess) return hip_fail(e, "batch MCMC set-up");
+  if ((e = unit->grid(mode, tiles, lds, &grid)) != hipSuccess) return hip_fail(e, "batch MCMC set-up");
   size_t q = 0;
   for (int phase = 0; phase < 2 && e == hipSuccess; ++phase) {
     size_t q_end = q;
@@ -1104,14 +1136,14 @@ int cdm_batch_mcmc(int nitems, const cdm_batch_item* items, int nburn, int njump
     auto issue = [&](hipStream_t st) -> hipError_t {
       hipError_t el = cdm::zero_ints(w->d_queues + q0, (int)(q_end - q0), st);
       for (size_t k = q0; k < q_end && el == hipSuccess; ++k)
-        el = unit->launch(cdm::kBatchMcmc, (unsigned char*)w->d_items + k * per, nitems, tiles, w->d_queues + k, lds,
+        el = unit->launch(mode, (unsigned char*)w->d_items + k * per, nitems, tiles, w->d_queues + k, lds,
                           0, grid, st);
       return el;
     };
     if (use_graph && q_end - q0 > 1) {
       std::vector<unsigned char> key;
-      const long long shape[8] = {(long long)(size_t)w->d_items, (long long)(size_t)w->d_queues, (long long)per,
-                                  nitems, tiles, (long long)lds, grid, (long long)((q0 << 32) | q_end)};
+      const long long shape[9] = {(long long)(size_t)w->d_items, (long long)(size_t)w->d_queues, (long long)per,
+                                  nitems, tiles, (long long)lds, grid, (long long)((q0 << 32) | q_end), mode};
       key_add(&key, shape);
       e = run_graph_cached(w->graphs[phase], &w->capture_stream, &g_batch_stats, key, s, issue);
     } else {
@@ -1121,6 +1153,59 @@ int cdm_batch_mcmc(int nitems, const cdm_batch_item* items, int nburn, int njump
   }
   if (e == hipSuccess) e = batch_note(nitems, items, w, s);
   return e == hipSuccess ? 0 : hip_fail(e, "batch MCMC");
+}
+
+int cdm_batch_levmar(int nitems, const cdm_batch_item* items, int max_iters, int marquardt, void* stream) {
+  const cdm::BatchOps* unit = nullptr;
+  std::vector<int> index;
+  long long total = 0;
+  int r = check_batch(nitems, items, false, &unit, &index, &total);
+  if (r) return r;
+  if (max_iters < 0) return fail("--iterLevMar must be >= 0");
+  std::vector<cdm_batch_item> rest;
+  for (int i = 0; i < nitems; ++i) {
+    const cdm_batch_item& it = items[i];
+    if (!fills_gpu(it.model, cdm::kLevMar, it.nmeas, it.nvox)) {
+      rest.push_back(it);
+      continue;
+    }
+    r = cdm_levmar(it.model, it.nvox, it.nmeas, it.meas, it.params, it.cfp, it.fixp, max_iters, marquardt, it.iters,
+                   it.cost, stream);
+    if (r) return fail("batch item " + std::to_string(i) + ": " + g_err);
+  }
+  if (rest.empty()) return 0;
+  return batch_levmar_persistent((int)rest.size(), rest.data(), max_iters, marquardt, stream);
+}
+
+int cdm_batch_mcmc(int nitems, const cdm_batch_item* items, int nburn, int njumps, int sampleevery,
+                   int update_proposal, int rician, unsigned long long seed, int jumps_per_launch, int use_graph,
+                   void* stream) {
+  const cdm::BatchOps* unit = nullptr;
+  std::vector<int> index;
+  long long total = 0;
+  int r = check_batch(nitems, items, true, &unit, &index, &total);
+  if (r) return r;
+  if (sampleevery <= 0) return fail("--sampleevery must be > 0");
+  if (njumps < sampleevery || njumps % sampleevery != 0)
+    return fail("--nj must be a positive multiple of --se");
+  if (jumps_per_launch < 0) return fail("jumps per launch must be >= 0");
+  for (int i = 0; i < nitems; ++i)
+    if (rician && !items[i].tau_samples) return fail("batch item " + std::to_string(i) + ": --rician needs tau_samples");
+  std::vector<cdm_batch_item> rest;
+  for (int i = 0; i < nitems; ++i) {
+    const cdm_batch_item& it = items[i];
+    if (!fills_gpu(it.model, cdm::kMcmc, it.nmeas, it.nvox)) {
+      rest.push_back(it);
+      continue;
+    }
+    r = mcmc_call(it.model, it.nvox, it.nmeas, it.meas, it.params, it.cfp, it.fixp, nburn, njumps, sampleevery,
+                  update_proposal, rician, seed, it.voxel_offset, it.samples, it.tau_samples, jumps_per_launch,
+                  use_graph != 0, (hipStream_t)stream);
+    if (r) return fail("batch item " + std::to_string(i) + ": " + g_err);
+  }
+  if (rest.empty()) return 0;
+  return batch_mcmc_persistent((int)rest.size(), rest.data(), nburn, njumps, sampleevery, update_proposal, rician,
+                               seed, jumps_per_launch, use_graph, stream);
 }
 
 int cdm_batch_graph_stats(long long* captures, long long* replays) {

@@ -81,7 +81,7 @@ const ModelOps* model_at(int i);
 // A batch unit compiles several plug-ins into one persistent kernel per
 // method: workgroups take voxel tiles (one block's worth of one item) from a
 // work queue until every item's tiles are done.
-enum BatchMode { kBatchLevMar = 0, kBatchMcmc = 1 };
+enum BatchMode { kBatchLevMar = 0, kBatchMcmc = 1, kBatchMcmcRician = 2 };
 
 struct BatchLaunchItem {
   int model;  // index in the unit's model list

@@ -238,12 +238,17 @@ int cdm_mlp_train(cdm_mlp* m, int n, int nmeas, const float* clean, const float*
  * Batched persistent launch.  No counterpart in the reference, which runs one
  * binary per model over its voxel sub-parts in turn (cudimot.cc:81-120).
  * Volumes of several models (each item: its own model handle, measurement
- * count and buffers, with the layouts above) are fitted by ONE launch per
- * method: workgroups take voxel tiles from a work queue, heaviest items
- * first, until all are done.  The models must be compiled together into a
- * batch unit (python -m cudimot_amd.build --batch NAME=A,B,...); each item
- * uses its model handle's bounds, priors, fixed parameters and tuned
- * voxel-group size.  The calls return when the work has finished. */
+ * count and buffers, with the layouts above) are fitted by one call per
+ * method.  Items too small to fill the GPU on their own share ONE
+ * persistent launch: workgroups take voxel tiles from a work queue,
+ * heaviest items first, until all are done.  An item whose voxels alone
+ * fill the GPU several times over (8 blocks of its voxel group per CU) runs
+ * through its model's own kernels at its tuned voxel group and register
+ * budget instead; the persistent kernel has one register allocation, the
+ * largest of its models'.  Results and draws are the same either way.  The
+ * models must be compiled together into a batch unit (python -m
+ * cudimot_amd.build --batch NAME=A,B,...); each item uses its model
+ * handle's bounds, priors, fixed parameters and tuned voxel-group size. */
 typedef struct cdm_batch_item {
   cdm_model* model; /* a model handle used by no other item of the batch */
   int nvox, nmeas;
@@ -254,7 +259,7 @@ typedef struct cdm_batch_item {
   int* iters;   /* LM outputs (nullable) */
   float* cost;
   float* samples;     /* MCMC outputs: [nvox][nparams][njumps / sampleevery] */
-  float* tau_samples; /* unused (Gaussian likelihood) */
+  float* tau_samples; /* MCMC with rician != 0: [nvox][njumps / sampleevery] */
   long long voxel_offset; /* MCMC random-stream offset */
 } cdm_batch_item;
 
@@ -262,19 +267,21 @@ typedef struct cdm_batch_item {
  * full length needed. */
 int cdm_batch_units(char* buf, int buflen);
 
-/* Levenberg_Marquardt<T>::run over every item in one launch.  The batched
- * calls are asynchronous on `stream`; their item tables and work-queue
- * counters live in a per-process workspace (one per method and device,
- * grown as needed), so a batched call first waits on the host for the
- * previous call of the same method on that device to finish. */
+/* Levenberg_Marquardt<T>::run over every item.  The batched calls are
+ * asynchronous on `stream`; the persistent launch's item tables and
+ * work-queue counters live in a per-process workspace (one per method and
+ * device, grown as needed), so a batched call first waits on the host for
+ * the previous call of the same method on that device to finish. */
 int cdm_batch_levmar(int nitems, const cdm_batch_item* items, int max_iters, int marquardt, void* stream);
 
 /* MCMC<T>::run over every item: one launch per jump block (jumps_per_launch
- * jumps, 0: a whole phase), each over all items; use_graph != 0 records a
- * phase's launches into a HIP graph, replayed by later calls of the same
- * shape (the workspace's graphs, cdm_batch_graph_stats).  Draws are those of
- * cdm_mcmc with the same seed and voxel offsets.  Gaussian likelihood only
- * (rician must be 0). */
+ * jumps, 0: a whole phase), each over all the persistent launch's items;
+ * use_graph != 0 records a phase's launches into a HIP graph, replayed by
+ * later calls of the same shape (the workspace's graphs,
+ * cdm_batch_graph_stats; for the items on their own kernels, the handle's
+ * graphs, cdm_model_graph_stats).  Draws are those of
+ * cdm_mcmc with the same seed and voxel offsets.  rician != 0: the Rician
+ * likelihood (MCMC.cu:799-847) for every item, each with its tau_samples. */
 int cdm_batch_mcmc(int nitems, const cdm_batch_item* items, int nburn, int njumps, int sampleevery,
                    int update_proposal, int rician, unsigned long long seed, int jumps_per_launch, int use_graph,
                    void* stream);

@@ -27,6 +27,11 @@ def test_batch_argument_errors_without_gpu():
         capi.batch_levmar([dict(model=b, **fake), dict(model=b, **fake)])
     with pytest.raises(capi.CdmError, match="at least one"):
         capi.batch_levmar([])
+    # the Rician likelihood needs every item's tau samples (checked before any device work)
+    items = [dict(model=capi.Model(name), meas=_Fake((10, 64)), params=8, cfp=8, fixp=8, samples=8)
+             for name, _, _ in BATCH]
+    with pytest.raises(capi.CdmError, match="needs tau_samples"):
+        capi.batch_mcmc(items, nburn=10, njumps=10, sampleevery=5, rician=True)
 
 
 class _Fake:
@@ -123,6 +128,102 @@ def test_batch_mcmc_equals_per_model_mcmc(blocks, graph):
     capi.sync()
     for (p, s), b in zip(ref, items):
         np.testing.assert_array_equal(host(s), host(b["samples"]))
+        np.testing.assert_array_equal(host(p), host(b["params"]))
+
+
+@pytest.mark.gpu
+def test_batch_item_that_fills_the_gpu_runs_its_own_kernels():
+    """A DiffusionTensor item of 140,000 voxels (2,188 blocks of its 4-lane
+    voxel groups, over 8 per CU) goes through the per-model kernels at its
+    tuned launch shape, the two small items through the persistent launch;
+    LM and MCMC results equal the per-model calls' either way, and the MCMC
+    jump-block graphs land in the big item's handle and in the batch
+    workspace respectively."""
+    import torch
+    from cudimot_amd import capi, synth
+    vols = _volumes(700)
+    n = 140000
+    cfp = synth.model_acquisition("DiffusionTensor")
+    t = synth.truth_for("DiffusionTensor", n, 71)
+    m0 = capi.Model("DiffusionTensor")
+    clean = torch.empty((n, cfp.shape[0]), device="cuda")
+    m0.predicted_signal(dev(t.reshape(n, -1, 1)), dev(cfp), None, clean)
+    rng = np.random.default_rng(72)
+    c = host(clean)
+    sig = (t[:, 0] / 20.0)[:, None]
+    meas = np.sqrt((c + sig * rng.normal(size=c.shape)) ** 2 + (sig * rng.normal(size=c.shape)) ** 2)
+    start = t.copy()
+    start[:, 1:] *= 1.0 + 0.05 * rng.normal(size=start[:, 1:].shape)
+    vols[0] = dict(name="DiffusionTensor", L=None, meas=dev(meas), cfp=dev(cfp), fixp=None, start=start, n=n)
+    ref_models = [capi.Model(v["name"]) for v in vols]
+    bat_models = [capi.Model(v["name"]) for v in vols]
+    for v, a, b in zip(vols, ref_models, bat_models):
+        for h in (a, b):
+            if v["L"]:  # the small items at fixed voxel groups (the persistent launch's are chosen for its total)
+                h.set_group(capi.LEVMAR, v["L"])
+                h.set_group(capi.MCMC, v["L"])
+    assert ref_models[0].get_group(capi.MCMC, cfp.shape[0], n) == 4
+    ref, items = [], []
+    off = 0
+    for v, m, mb in zip(vols, ref_models, bat_models):
+        P = m.nparams
+        p = dev(v["start"])
+        cost = torch.empty(v["n"], device="cuda")
+        m.levmar(v["meas"], p, v["cfp"], v["fixp"], max_iters=50, cost=cost)
+        s = torch.empty((v["n"], P, 2), device="cuda")
+        m.set_mcmc_blocks(20, use_graph=True)
+        m.mcmc(v["meas"], p, v["cfp"], v["fixp"], s, None, nburn=40, njumps=20, sampleevery=10, seed=5,
+               voxel_offset=off)
+        ref.append((p, cost, s))
+        items.append(dict(model=mb, meas=v["meas"], params=dev(v["start"]), cfp=v["cfp"], fixp=v["fixp"],
+                          cost=torch.empty(v["n"], device="cuda"), samples=torch.empty((v["n"], P, 2), device="cuda"),
+                          voxel_offset=off))
+        off += v["n"]
+    before = capi.batch_graph_stats()
+    capi.batch_levmar(items, max_iters=50)
+    capi.batch_mcmc(items, nburn=40, njumps=20, sampleevery=10, seed=5, jumps_per_launch=20, use_graph=True)
+    capi.sync()
+    for (p, cost, s), b in zip(ref, items):
+        np.testing.assert_array_equal(host(cost), host(b["cost"]))
+        np.testing.assert_array_equal(host(s), host(b["samples"]))
+        np.testing.assert_array_equal(host(p), host(b["params"]))
+    assert bat_models[0].graph_stats()[0] == 1  # the burn-in's two blocks, recorded on the handle
+    assert bat_models[1].graph_stats() == (0, 0) and bat_models[2].graph_stats() == (0, 0)
+    assert sum(capi.batch_graph_stats()) == sum(before) + 1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("blocks,graph", [(0, False), (25, True)])
+def test_batch_rician_mcmc_equals_per_model_mcmc(blocks, graph):
+    """The Rician likelihood (MCMC.cu:799-847) in the batched launch: samples,
+    tau samples and final parameters equal the per-model Rician MCMC's."""
+    import torch
+    from cudimot_amd import capi
+    vols = _volumes(600)
+    ref_models, bat_models = _models(vols, capi.MCMC), _models(vols, capi.MCMC)
+    ref, items = [], []
+    off = 0
+    for v, m, mb in zip(vols, ref_models, bat_models):
+        P = m.nparams
+        p = dev(v["start"])
+        s = torch.empty((v["n"], P, 5), device="cuda")
+        tau = torch.empty((v["n"], 5), device="cuda")
+        m.mcmc(v["meas"], p, v["cfp"], v["fixp"], s, tau, nburn=50, njumps=50, sampleevery=10, rician=True, seed=91,
+               voxel_offset=off)
+        ref.append((p, s, tau))
+        pb = dev(v["start"])
+        sb = torch.empty((v["n"], P, 5), device="cuda")
+        tb = torch.empty((v["n"], 5), device="cuda")
+        items.append(dict(model=mb, meas=v["meas"], params=pb, cfp=v["cfp"], fixp=v["fixp"], samples=sb,
+                          tau_samples=tb, voxel_offset=off))
+        off += v["n"]
+    capi.batch_mcmc(items, nburn=50, njumps=50, sampleevery=10, seed=91, jumps_per_launch=blocks, use_graph=graph,
+                    rician=True)
+    capi.sync()
+    for (p, s, tau), b in zip(ref, items):
+        assert np.all(np.isfinite(host(tau))) and np.all(host(tau) > 0)
+        np.testing.assert_array_equal(host(s), host(b["samples"]))
+        np.testing.assert_array_equal(host(tau), host(b["tau_samples"]))
         np.testing.assert_array_equal(host(p), host(b["params"]))
 
 

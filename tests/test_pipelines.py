@@ -103,6 +103,8 @@ def test_learned_pipeline_matches_gridsearch_pipeline():
     t, cfp, s0, meas = _volume(4096, 901)
     pl, cl, fl = pipelines.bingham_learned_lm(meas, cfp, s0, _net())
     pg, cg = pipelines.bingham_gridsearch_lm(meas, cfp, s0)
+    print("learned / GridSearch RMSE:", [round(_rmse(pl, t, k) / _rmse(pg, t, k), 4) for k in range(4)],
+          "median cost ratio", np.median(host(cl)) / np.median(host(cg)), "fallback", host(fl).mean())
     for k in (0, 1):
         assert _rmse(pl, t, k) <= 1.15 * _rmse(pg, t, k), (k, _rmse(pl, t, k), _rmse(pg, t, k))
     for k in (2, 3):
