@@ -53,6 +53,13 @@ constexpr int cdm_ntri = cdm::tri_size(NPARAMS);
 //                              vc, mc, mvc) fills mvc as Meas_Voxel_Cache and
 //                              returns the signal of Predicted_Signal_MV,
 //                              sharing the work the two have in common
+//   CDM_MEAS_VOXEL_CACHE_PARTIAL  Predicted_Signal_MV_Update(npar, P, CFP,
+//                              FixP, vc, mc, cur, mvc, changed) fills mvc
+//                              after a change of parameter `changed` alone,
+//                              recomputing only the entries that depend on
+//                              it and copying the others from cur (all of
+//                              them when changed < 0), and returns the
+//                              signal of Predicted_Signal_MV
 //   CDM_VOXEL_CACHE_JOBS_DEPENDS b  the jobs of CDM_VOXEL_CACHE_JOBS depend
 //                              only on the parameters in b (a subset of
 //                              CDM_VOXEL_CACHE_DEPENDS): an MCMC step in
@@ -805,45 +812,51 @@ __device__ __forceinline__ void cdm_lm_jacobian_columns(const cdm_geom& g, const
     h = min_gpu(max_gpu(h, (cdm_jt)::cdm_fd_step<cdm_jt>::lo), (cdm_jt)::cdm_fd_step<cdm_jt>::hi);
     const cdm_jt up = pi + h * (cdm_jt)0.5;
     const cdm_jt dn = up - (cdm_jt)2 * (h * (cdm_jt)0.5);
-    cdm_jt pu[cdm_np], pd[cdm_np];
-#pragma unroll
-    for (int k = 0; k < cdm_np; ++k) {
-      pu[k] = k == i ? up : pj[k];
-      pd[k] = k == i ? dn : pj[k];
-    }
     const bool dep = (cdm_vc_dep >> i) & 1u, jdep = (cdm_vcj_dep >> i) & 1u;
-    CDM_F64 vcu[cdm_vc], vcd[cdm_vc];
-    if (!dep) {
-#pragma unroll
-      for (int c = 0; c < cdm_vc; ++c) vcu[c] = vcd[c] = vc0[c];
-    } else if (jdep) {
-      // the two probes' jobs, pair t = probe * jobs + job, one per lane
+    // the two probes' jobs, pair t = probe * jobs + job, one per lane
+    CDM_F64 mine = 0.0;
+    if (dep && jdep) {
       constexpr int T = 2 * cdm_vc_jobs;
       static_assert(T <= L, "one round of probe jobs");
       const int t = lane < T ? lane : 0;
-      // element-wise select: a pointer select between two register arrays
-      // would place both in scratch memory
+      const cdm_jt pt = t < cdm_vc_jobs ? up : dn;
       cdm_jt q[cdm_np];
 #pragma unroll
-      for (int k = 0; k < cdm_np; ++k) q[k] = t < cdm_vc_jobs ? pu[k] : pd[k];
-      const CDM_F64 mine = Voxel_Cache_Job(t % cdm_vc_jobs, cdm_np, q, fxj);
-      CDM_F64 ju[cdm_vc_jobs], jd[cdm_vc_jobs];
-#pragma unroll
-      for (int j = 0; j < cdm_vc_jobs; ++j) {
-        ju[j] = cdm::Group<L>::bcast(mine, j);
-        jd[j] = cdm::Group<L>::bcast(mine, cdm_vc_jobs + j);
-      }
-      Voxel_Cache_Combine(cdm_np, pu, fxj, ju, vcu);
-      Voxel_Cache_Combine(cdm_np, pd, fxj, jd, vcd);
-    } else {
-      Voxel_Cache_Combine(cdm_np, pu, fxj, jr0, vcu);
-      Voxel_Cache_Combine(cdm_np, pd, fxj, jr0, vcd);
+      for (int k = 0; k < cdm_np; ++k) q[k] = k == i ? pt : pj[k];
+      mine = Voxel_Cache_Job(t % cdm_vc_jobs, cdm_np, q, fxj);
     }
-    for (int m = lane; m < g.nmeas; m += L) {
-      cdm_jt* cm = rowj.at(s_cfp, m, g.cfp_ts);
-      const cdm_jt su = cdm_signal(pu, cm, fxj, vcu, cdm_mc_row(s_mc, m));
-      const cdm_jt sd = cdm_signal(pd, cm, fxj, vcd, cdm_mc_row(s_mc, m));
-      rows[m * np1 + i] = (CDM_F64)(cdm_lm_acc)(((su - sd) / h) * di);
+    // the probes one after the other, so that one probe's cache is live at a
+    // time: the upper probe's signals go to the column's LDS entries (as
+    // doubles, exact for the signal's type), the lower probe's pass forms
+    // the quotient from them
+#pragma unroll 1
+    for (int probe = 0; probe < 2; ++probe) {
+      const cdm_jt pp = probe == 0 ? up : dn;
+      cdm_jt q[cdm_np];
+#pragma unroll
+      for (int k = 0; k < cdm_np; ++k) q[k] = k == i ? pp : pj[k];
+      CDM_F64 vc[cdm_vc];
+      if (!dep) {
+#pragma unroll
+        for (int c = 0; c < cdm_vc; ++c) vc[c] = vc0[c];
+      } else if (jdep) {
+        CDM_F64 jq[cdm_vc_jobs];
+#pragma unroll
+        for (int j = 0; j < cdm_vc_jobs; ++j) jq[j] = cdm::Group<L>::bcast(mine, probe * cdm_vc_jobs + j);
+        Voxel_Cache_Combine(cdm_np, q, fxj, jq, vc);
+      } else {
+        Voxel_Cache_Combine(cdm_np, q, fxj, jr0, vc);
+      }
+      for (int m = lane; m < g.nmeas; m += L) {
+        cdm_jt* cm = rowj.at(s_cfp, m, g.cfp_ts);
+        const cdm_jt sq = cdm_signal(q, cm, fxj, vc, cdm_mc_row(s_mc, m));
+        if (probe == 0) {
+          rows[m * np1 + i] = (CDM_F64)sq;
+        } else {
+          const cdm_jt su = (cdm_jt)rows[m * np1 + i];
+          rows[m * np1 + i] = (CDM_F64)(cdm_lm_acc)(((su - sq) / h) * di);
+        }
+      }
     }
   }
 }
@@ -1312,7 +1325,8 @@ __device__ __forceinline__ double cdm_neg_loglik(const cdm_geom& g, const float*
 template <int L, bool RICIAN, int MODE>
 __device__ __forceinline__ double cdm_neg_loglik_mv(const cdm_geom& g, const float* s_cfp, const CDM_F64* s_mc,
                                                     const float* mv, const double* p, double tau, cdm_tf* fx,
-                                                    const CDM_F64* vc, cdm_tf* row, int lane) {
+                                                    const CDM_F64* vc, cdm_tf* row, int lane,
+                                                    const cdm_tf* row_cur = nullptr, int changed = -1) {
 #ifdef CDM_MEAS_VOXEL_CACHE
   cdm_tf pf[cdm_np];
 #pragma unroll
@@ -1325,7 +1339,12 @@ __device__ __forceinline__ double cdm_neg_loglik_mv(const cdm_geom& g, const flo
     cdm_tf mvc[cdm_mvc];
     cdm_tf s;
     if constexpr (MODE == 2) {
-#ifdef CDM_MEAS_VOXEL_CACHE_FILL
+#if defined(CDM_MEAS_VOXEL_CACHE_PARTIAL)
+      cdm_tf cur[cdm_mvc];
+#pragma unroll
+      for (int k = 0; k < cdm_mvc; ++k) cur[k] = changed >= 0 ? row_cur[m * cdm_mvc + k] : (cdm_tf)0;
+      s = Predicted_Signal_MV_Update(cdm_np, pf, cfpm, fx, vc, mcm, cur, mvc, changed);
+#elif defined(CDM_MEAS_VOXEL_CACHE_FILL)
       s = Predicted_Signal_MV_Fill(cdm_np, pf, cfpm, fx, vc, mcm, mvc);
 #else
       Meas_Voxel_Cache(cdm_np, pf, cfpm, fx, vc, mcm, mvc);
@@ -1629,7 +1648,7 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
         double like;
         if constexpr (cdm_mvc > 0) {
           like = mdep ? cdm_neg_loglik_mv<L, RICIAN, 2>(a.g, s_cfp, s_mc, mv, p, tau, fx, vcu,
-                                                        mvc_rows + (1 - cur) * rs, lane)
+                                                        mvc_rows + (1 - cur) * rs, lane, mvc_rows + cur * rs, i)
                       : cdm_neg_loglik_mv<L, RICIAN, 1>(a.g, s_cfp, s_mc, mv, p, tau, fx, vcu, mvc_rows + cur * rs,
                                                         lane);
         } else {

@@ -11,7 +11,9 @@
 // (five 1F1 evaluations, the cache's jobs, depending on kappa and beta) and
 // the local frame (depending on the three angles) are computed once per
 // parameter vector instead of once per measurement; a step or probe in
-// fintra or an angle recombines the kept 1F1 values.
+// fintra or an angle recombines the kept 1F1 values.  MCMC keeps each fan's
+// signal per measurement (Fk): a step in fiso or fFan2 recombines the kept
+// signals, a step in one fan's parameters recomputes that fan's only.
 #include "../NODDI_Bingham/bingham_math.h"
 
 #define CDM_VOXEL_CACHE 26                // per fan: lnc4, dpar, dperp1, dperp2, frame (9)
@@ -21,6 +23,9 @@
 #define CDM_NUMERICAL_MASK 0x3FFFu        // all 14, as the reference
 #define CDM_MEAS_CACHE 1                  // exp(-b Diso) per measurement
 #define CDM_LM_JACOBIAN_COLUMNS 1         // 64-lane LM: the Jacobian column by column
+#define CDM_MEAS_VOXEL_CACHE 2            // F1, F2 per measurement
+#define CDM_MEAS_VOXEL_CACHE_DEPENDS 0x3FFCu
+#define CDM_MEAS_VOXEL_CACHE_PARTIAL 1    // a step in one fan recomputes that fan
 
 constexpr int b2f_fan = 13;  // cache values per fan
 
@@ -73,6 +78,36 @@ MACRO T Predicted_Signal_Cached(int npar, T* P, T* CFP, T* FixP, const CDM_F64* 
   const T fan2 = b2f_fan_signal(1, P, CFP, vc);
   const T aniso = ((T)1 - P[1]) * fan1 + P[1] * fan2;
   return FixP[0] * (((T)1 - P[0]) * aniso + P[0] * (T)mc[0]);
+}
+
+// The signal from the kept fan signals: the arithmetic of
+// Predicted_Signal_Cached after its two fan evaluations.
+MACRO T Predicted_Signal_MV(int npar, T* P, T* CFP, T* FixP, const CDM_F64* vc, const CDM_F64* mc, const T* mvc) {
+  const T fan1 = mvc[0];
+  const T fan2 = mvc[1];
+  const T aniso = ((T)1 - P[1]) * fan1 + P[1] * fan2;
+  return FixP[0] * (((T)1 - P[0]) * aniso + P[0] * (T)mc[0]);
+}
+
+MACRO void Meas_Voxel_Cache(int npar, T* P, T* CFP, T* FixP, const CDM_F64* vc, const CDM_F64* mc, T* mvc) {
+  mvc[0] = b2f_fan_signal(0, P, CFP, vc);
+  mvc[1] = b2f_fan_signal(1, P, CFP, vc);
+}
+
+// After a change of parameter `changed` alone: fan 1's parameters are
+// P[2..7], fan 2's P[8..13]; fiso and fFan2 move neither fan's signal.
+MACRO T Predicted_Signal_MV_Update(int npar, T* P, T* CFP, T* FixP, const CDM_F64* vc, const CDM_F64* mc,
+                                   const T* cur, T* mvc, int changed) {
+  const bool all = changed < 0;
+  if (all || (changed >= 2 && changed < 8))
+    mvc[0] = b2f_fan_signal(0, P, CFP, vc);
+  else
+    mvc[0] = cur[0];
+  if (all || changed >= 8)
+    mvc[1] = b2f_fan_signal(1, P, CFP, vc);
+  else
+    mvc[1] = cur[1];
+  return Predicted_Signal_MV(npar, P, CFP, FixP, vc, mc, mvc);
 }
 
 // No analytic entries: the engine forms all 14 by central differences.
