@@ -60,6 +60,8 @@ def lib():
             "cdm_levmar": (i, [vp, i, i, vp, vp, vp, vp, i, i, vp, vp, vp]),
             "cdm_mcmc": (i, [vp, i, i, vp, vp, vp, vp, i, i, i, i, i, u64, i64, vp, vp, vp]),
             "cdm_mcmc_noalloc": (i, [vp, i, i, vp, vp, vp, vp, i, i, i, i, i, u64, i64, vp, vp, vp]),
+            "cdm_model_set_debug": (i, [vp, i, i]),
+            "cdm_model_debug_trace": (i, [vp, i, vp, i, ip, vp]),
             "cdm_predicted_signal": (i, [vp, i, i, i, vp, vp, vp, vp, vp]),
             "cdm_bic_aic": (i, [vp, i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp]),
             "cdm_rician_clean": (i, [i, i, vp]),
@@ -199,6 +201,23 @@ class Model:
 
     def set_mcmc_blocks(self, jumps_per_launch, use_graph=False):
         check(lib().cdm_model_set_mcmc_blocks(self.h, int(jumps_per_launch), 1 if use_graph else 0))
+
+    def set_debug(self, voxel, max_records=4096):
+        """--debug=<voxel>: GridSearch, LM and MCMC record that voxel's state
+        (voxel < 0: off)."""
+        check(lib().cdm_model_set_debug(self.h, int(voxel), int(max_records)))
+
+    def debug_trace(self, method, stream=None):
+        """The last call's records of `method` (GRIDSEARCH, LEVMAR, MCMC):
+        array [n][4 + nparams] (index, three values, the parameters)."""
+        import numpy as np
+        n = ctypes.c_int()
+        check(lib().cdm_model_debug_trace(self.h, method, None, 0, ctypes.byref(n), _stream(stream)))
+        out = np.zeros((max(n.value, 0), 4 + self.nparams))
+        if n.value:
+            check(lib().cdm_model_debug_trace(self.h, method, out.ctypes.data_as(ctypes.c_void_p), n.value,
+                                              ctypes.byref(n), _stream(stream)))
+        return out
 
     def graph_stats(self):
         """(graphs recorded, graphs replayed from the handle's cache)."""

@@ -356,19 +356,56 @@ void fit(const Options& o) {
   if (!fixp.empty()) d_fixp.up(fixp);
   const float* cfp_p = cfp.empty() ? nullptr : d_cfp.p;
   const float* fixp_p = fixp.empty() ? nullptr : d_fixp.p;
-  if (!o.gridSearch.empty())
+  // --debug=<voxel>: the methods' records of that voxel, printed after each
+  // method (GridSearch.cu:284-286 reads the option the same way)
+  const int debug_voxel = o.debug.empty() ? -1 : std::atoi(o.debug.c_str());
+  if (debug_voxel >= 0) {
+    const int cap = std::max({1, (o.runMCMC ? std::max(o.nburn, 1) + o.njumps : 0), 32 * (o.iterLevMar + 1)});
+    check(cdm_model_set_debug(m.h, debug_voxel, cap), "--debug");
+  }
+  auto print_debug = [&](int method, const char* title) {
+    if (debug_voxel < 0 || debug_voxel >= nvox) return;
+    int n = 0;
+    check(cdm_model_debug_trace(m.h, method, nullptr, 0, &n, nullptr), "--debug");
+    std::vector<double> rec((size_t)std::max(n, 1) * (4 + P));
+    check(cdm_model_debug_trace(m.h, method, rec.data(), n, &n, nullptr), "--debug");
+    std::printf("\n ----- %s GPU algorithm: voxel %i -----\n", title, debug_voxel);
+    for (int k = 0; k < n; ++k) {
+      const double* r = &rec[(size_t)k * (4 + P)];
+      if (method == CDM_GRIDSEARCH)
+        std::printf("Kept combination %d of %d, cost %f\n", (int)r[0], (int)r[2], r[1]);
+      else if (method == CDM_LEVMAR && k == 0)
+        std::printf("Initial cost %f\n", r[1]);
+      else if (method == CDM_LEVMAR)
+        std::printf("Iteration %d: cost %f, lambda %g, %s\n", (int)r[0], r[1], r[2],
+                    r[3] != 0.0 ? "accepted" : "rejected");
+      else
+        std::printf("Jump %d: energy %f, accepted since the last proposal update %d%s\n", (int)r[0], r[1],
+                    (int)r[3], o.rician ? (", tau " + std::to_string(r[2])).c_str() : "");
+      for (int i = 0; i < P; ++i)
+        std::printf("%s Parameter[%i]: %f\n", method == CDM_LEVMAR && k == 0 ? "Initial" : "", i, r[4 + i]);
+    }
+    std::printf("--------------------------------------------------------\n");
+  };
+  if (!o.gridSearch.empty()) {
     check(cdm_gridsearch(m.h, nvox, nmeas, d_meas.p, d_params.p, cfp_p, fixp_p, nullptr, nullptr), "GridSearch");
-  if (!o.no_LevMar)
+    print_debug(CDM_GRIDSEARCH, "GridSearch");
+  }
+  if (!o.no_LevMar) {
     check(cdm_levmar(m.h, nvox, nmeas, d_meas.p, d_params.p, cfp_p, fixp_p, o.iterLevMar, !o.no_Marquardt, nullptr,
                      nullptr, nullptr),
           "Levenberg-Marquardt");
+    print_debug(CDM_LEVMAR, "Levenberg_Marquardt");
+  }
   Dev<float> d_samples(o.runMCMC ? (size_t)nvox * P * nsamples : 0);
   Dev<float> d_tau(o.runMCMC && o.rician ? (size_t)nvox * nsamples : 0);
-  if (o.runMCMC)
+  if (o.runMCMC) {
     check(cdm_mcmc(m.h, nvox, nmeas, d_meas.p, d_params.p, cfp_p, fixp_p, o.nburn, o.njumps, o.sampleevery,
                    !o.no_updateproposal, o.rician, (unsigned long long)o.seed, voxel_offset, d_samples.p, d_tau.p,
                    nullptr),
           "MCMC");
+    print_debug(CDM_MCMC, "MCMC");
+  }
   // Predicted signal and BIC/AIC from the parameters (no MCMC) or the
   // samples (Parameters.cu:320-358).
   const float* src = o.runMCMC ? d_samples.p : d_params.p;

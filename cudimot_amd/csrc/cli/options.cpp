@@ -48,7 +48,9 @@ const Spec kSpecs[] = {
     {"--FixP", kStr, false, "File with a list of NIfTI files for specifying the fixed parameters of the model"},
     {"--fixed", kStr, false, "List of the Ids of the fixed parameters separated by commas. For example: --fixed=2,4"},
     {"--init_params", kStr, false, "File with a list of NIfTI files for the initialization of the model parameters"},
-    {"--debug", kStr, false, "Specify a voxel for debugging (accepted and ignored)"},
+    {"--debug", kStr, false,
+     "Specify a voxel for debugging. Some variables at certain steps of the algorithms will be printed (use few "
+     "iterations)"},
     {"--BIC_AIC", kBool, false, "Calculate Bayesian and Akaike Information Criteria at the end"},
     {"--priors", kStr, false, "File with parameters information (initialization, bounds and priors)"},
     {"--model", kStr, false, "Model plug-in (when the program name does not carry it)"},

@@ -14,6 +14,9 @@ static cdm_consts op_consts(const cdm::Consts& in) {
     c.pb[i] = in.pb[i];
     c.fixed[i] = in.fixed[i];
   }
+  c.dbg_voxel = in.dbg_voxel;
+  c.dbg_cap = in.dbg_cap;
+  c.dbg = in.dbg;
   return c;
 }
 

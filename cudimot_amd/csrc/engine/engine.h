@@ -178,7 +178,36 @@ struct cdm_consts {
   int prior_types[cdm_np];
   float pa[cdm_np], pb[cdm_np];
   int fixed[cdm_np];
+  int dbg_voxel, dbg_cap;  // --debug=<voxel> records (dbg null: off)
+  double* dbg;
 };
+
+// --debug=<voxel> (cudimotoptions.h:160-161; the reference prints the
+// voxel's state from inside its kernels, e.g. Levenberg_Marquardt.cu:482-501,
+// MCMC.cu:437-456): the methods write state records of that voxel to a
+// device buffer instead, which the host prints.  Region r (0 GridSearch,
+// 1 Levenberg-Marquardt, 2 MCMC) of c.dbg holds the number of records
+// written, then c.dbg_cap records of cdm_dbg_width doubles: an index, three
+// values and the parameters.  Record `slot` is written by every lane of the
+// voxel's group (the same values to the same addresses: no branch on the
+// lane index).
+constexpr int cdm_dbg_width = 4 + cdm_np;
+__host__ __device__ constexpr size_t cdm_dbg_region_doubles(int cap) { return 8 + (size_t)cap * cdm_dbg_width; }
+
+template <class TP>
+__device__ __forceinline__ void cdm_dbg_record(const cdm_consts& c, int region, int slot, double a0, double a1,
+                                               double a2, double a3, const TP* p) {
+  if (slot < 0 || slot >= c.dbg_cap) return;
+  double* base = c.dbg + (size_t)region * cdm_dbg_region_doubles(c.dbg_cap);
+  double* r = base + 8 + (size_t)slot * cdm_dbg_width;
+  r[0] = a0;
+  r[1] = a1;
+  r[2] = a2;
+  r[3] = a3;
+#pragma unroll
+  for (int k = 0; k < cdm_np; ++k) r[4 + k] = (double)p[k];
+  base[0] = (double)(slot + 1);
+}
 
 struct cdm_geom {
   int nvox, nmeas, cfp_ts, fixp_ts;
@@ -357,6 +386,7 @@ __global__ void __launch_bounds__(cdm_block) cdm_gridsearch_kernel(cdm_gs_args a
   }
 
   int chosen = best_i[0];
+  double chosen_cost = best_c[0];
   if constexpr (sizeof(cdm_tf) != sizeof(double)) {
     // Error band of the FP32 cost (rounding of the forward model and of the
     // residual sum), generous by a factor ~10.
@@ -406,6 +436,7 @@ __global__ void __launch_bounds__(cdm_block) cdm_gridsearch_kernel(cdm_gs_args a
       }
     }
     chosen = bestd_i;
+    chosen_cost = bestd;
   }
   if (chosen < 0) {
     if (lane == 0 && a.kept_out) a.kept_out[v] = -1;
@@ -414,6 +445,21 @@ __global__ void __launch_bounds__(cdm_block) cdm_gridsearch_kernel(cdm_gs_args a
   if (lane == 0) {
     for (int k = 0; k < a.ngp; ++k) a.params[(size_t)v * cdm_np + a.gparams[k]] = a.grid_f[(size_t)chosen * a.ngp + k];
     if (a.kept_out) a.kept_out[v] = a.comb_id[chosen];
+    if (a.c.dbg && v == a.c.dbg_voxel) {
+      // the kept combination, its cost (the FP64 rescore's, or the screen's
+      // when the plug-in is double), the valid combinations, the parameters
+      double pk[cdm_np];
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i) pk[i] = (double)base[i];
+      for (int k = 0; k < a.ngp; ++k) {
+#pragma unroll
+        for (int i = 0; i < cdm_np; ++i)
+          if (i == a.gparams[k]) pk[i] = (double)a.grid_f[(size_t)chosen * a.ngp + k];
+      }
+      double kc = best_c[0];
+      if constexpr (sizeof(cdm_tf) != sizeof(double)) kc = chosen_cost;
+      cdm_dbg_record(a.c, 0, 0, (double)a.comb_id[chosen], kc, (double)a.ncombs, 0.0, pk);
+    }
   }
 }
 
@@ -952,6 +998,11 @@ __device__ __forceinline__ void cdm_levmar_voxel(const cdm_lm_args& a, int marqu
   double lambda = 0.1, olambda = 0.0;
   bool success = true;
   int iter = 0, nsucc = 0;
+  // --debug: record 0 the start, record k the state after attempt k
+  // (attempt, cost, lambda after it, accepted, parameters)
+  const bool dbg = a.c.dbg && v == a.c.dbg_voxel;
+  int dbg_slot = 0;
+  if (dbg) cdm_dbg_record(a.c, 1, dbg_slot++, 0.0, pcf, lambda, 1.0, p);
   cdm_lm_acc hs[cdm_ntri];  // 2 J^T J, packed lower triangle (undamped)
   double grad[cdm_np];  // 2 J^T r
   double diag[cdm_np];  // current damped diagonal
@@ -1138,6 +1189,10 @@ __device__ __forceinline__ void cdm_levmar_voxel(const cdm_lm_args& a, int marqu
       lambda = lambda * 10.0;
       if (lambda > cdm_lm_ltol) end = true;
       cdm_transform(a.c, p, t);
+    }
+    if (dbg) {
+      cdm_dbg_record(a.c, 1, dbg_slot, (double)dbg_slot, success ? pcf : ncf, lambda, success ? 1.0 : 0.0, p);
+      ++dbg_slot;
     }
     if (end) break;
   }
@@ -1679,6 +1734,14 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
         ++nrej[i];
         p[i] = old;
       }
+    }
+    // --debug: one record per jump (jump, energy, tau, acceptances since the
+    // last proposal update, parameters)
+    if (a.c.dbg && v == a.c.dbg_voxel) {
+      int acc = 0;
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i) acc += nacc[i];
+      cdm_dbg_record(a.c, 2, (int)jump, (double)jump, energy, tau, (double)acc, p);
     }
     // every lane of the group holds the same state and stores it (no branch
     // on the lane index inside the jump loop)

@@ -78,6 +78,11 @@ struct cdm_model {
   hipStream_t last_stream = nullptr;
   hipEvent_t last_done = nullptr;
   bool has_last = false;
+  // --debug=<voxel>: device records of that voxel, three regions (GridSearch,
+  // LM, MCMC) of debug_cap records each (d_debug null: off)
+  int debug_voxel = -1;
+  int debug_cap = 0;
+  double* d_debug = nullptr;
 };
 
 namespace {
@@ -117,7 +122,21 @@ cdm::Consts make_consts(const cdm_model* m) {
     c.pb[i] = (float)s.pb[i];
     c.fixed[i] = s.fixed[i];
   }
+  c.dbg_voxel = m->debug_voxel;
+  c.dbg_cap = m->debug_cap;
+  c.dbg = m->d_debug;
   return c;
+}
+
+// Doubles of one --debug region: the record count, then debug_cap records of
+// (index, three values, the parameters).
+size_t debug_region_doubles(const cdm_model* m) { return 8 + (size_t)m->debug_cap * (4 + m->ops->nparams); }
+
+// Clears a method's --debug records before its launches (a fill kernel: the
+// count is the region's first double, zero bits).
+hipError_t debug_reset(const cdm_model* m, int region, hipStream_t s) {
+  if (!m->d_debug) return hipSuccess;
+  return cdm::zero_ints(reinterpret_cast<int*>(m->d_debug + (size_t)region * debug_region_doubles(m)), 2, s);
 }
 
 int tsize(int n, const int* sizes) {
@@ -459,7 +478,52 @@ void cdm_model_close(cdm_model* m) {
   for (GraphCache& g : m->mcmc_graphs) g.clear();
   if (m->capture_stream) (void)hipStreamDestroy(m->capture_stream);
   if (m->last_done) (void)hipEventDestroy(m->last_done);
+  if (m->d_debug) (void)hipFree(m->d_debug);
   delete m;
+}
+
+int cdm_model_set_debug(cdm_model* m, int voxel, int max_records) {
+  if (!m) return fail("null model handle");
+  if (voxel >= 0 && max_records <= 0) return fail("--debug needs room for at least one record");
+  if (m->has_last && m->last_done) (void)hipEventSynchronize(m->last_done);
+  if (m->d_debug) (void)hipFree(m->d_debug);
+  m->d_debug = nullptr;
+  m->debug_voxel = -1;
+  m->debug_cap = 0;
+  if (voxel < 0) return 0;
+  m->debug_cap = max_records;
+  const size_t bytes = 3 * debug_region_doubles(m) * sizeof(double);
+  hipError_t e = hipMalloc((void**)&m->d_debug, bytes);
+  if (e == hipSuccess) e = hipMemset(m->d_debug, 0, bytes);
+  if (e != hipSuccess) {
+    if (m->d_debug) (void)hipFree(m->d_debug);
+    m->d_debug = nullptr;
+    m->debug_cap = 0;
+    return hip_fail(e, "--debug records");
+  }
+  m->debug_voxel = voxel;
+  return 0;
+}
+
+int cdm_model_debug_trace(cdm_model* m, int method, double* out, int max_records, int* nrecords, void* stream) {
+  if (!m || !nrecords) return fail("bad arguments");
+  *nrecords = 0;
+  const int region = method == cdm::kGridSearch ? 0 : method == cdm::kLevMar ? 1 : method == cdm::kMcmc ? 2 : -1;
+  if (region < 0) return fail("--debug records exist for GridSearch, Levenberg-Marquardt and MCMC only");
+  if (!m->d_debug) return 0;
+  const size_t n = debug_region_doubles(m);
+  std::vector<double> host(n);
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipMemcpyAsync(host.data(), m->d_debug + (size_t)region * n, n * sizeof(double),
+                                hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "--debug records");
+  const int have = std::min((int)host[0], m->debug_cap);
+  const int w = 4 + m->ops->nparams;
+  const int k = std::min(have, std::max(max_records, 0));
+  if (out && k > 0) std::memcpy(out, host.data() + 8, (size_t)k * w * sizeof(double));
+  *nrecords = have;
+  return 0;
 }
 
 int cdm_model_graph_stats(const cdm_model* m, long long* captures, long long* replays) {
@@ -582,10 +646,11 @@ int cdm_gridsearch(cdm_model* m, int nvox, int nmeas, const float* meas, float* 
   if (nvox == 0) return 0;
   const cdm::Geom g = make_geom(m, nvox, nmeas, meas, cfp, fixp);
   const cdm::Consts c = make_consts(m);
-  const hipError_t e = m->ops->gridsearch(group_for(m, cdm::kGridSearch, nmeas, nvox), g, params,
-                                          (int)m->spec.grid_params.size(), m->spec.grid_params.data(),
-                                          m->ncombs_valid, m->d_grid_f, m->d_grid_d, m->d_comb_id, c, kept,
-                                          (hipStream_t)stream);
+  hipError_t e = debug_reset(m, 0, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "GridSearch --debug records");
+  e = m->ops->gridsearch(group_for(m, cdm::kGridSearch, nmeas, nvox), g, params, (int)m->spec.grid_params.size(),
+                         m->spec.grid_params.data(), m->ncombs_valid, m->d_grid_f, m->d_grid_d, m->d_comb_id, c, kept,
+                         (hipStream_t)stream);
   return e == hipSuccess ? 0 : hip_fail(e, "GridSearch launch");
 }
 
@@ -610,6 +675,7 @@ int cdm_levmar(cdm_model* m, int nvox, int nmeas, const float* meas, float* para
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = order_after_last(m, s);
   if (e != hipSuccess) return hip_fail(e, "Levenberg-Marquardt stream order");
+  if ((e = debug_reset(m, 1, s)) != hipSuccess) return hip_fail(e, "Levenberg-Marquardt --debug records");
   e = m->ops->levmar(group_for(m, cdm::kLevMar, nmeas, nvox), occ_for(m, cdm::kLevMar, nmeas), g, params, max_iters,
                      marquardt ? 1 : 0, make_consts(m), iters, cost, m->d_queue, ncu, s);
   if (e != hipSuccess) return hip_fail(e, "Levenberg-Marquardt launch");
@@ -720,6 +786,7 @@ int mcmc_call(cdm_model* m, int nvox, int nmeas, const float* meas, float* param
   if (r) return r;
   hipError_t e = order_after_last(m, s);
   if (e != hipSuccess) return hip_fail(e, "MCMC stream order");
+  if ((e = debug_reset(m, 2, s)) != hipSuccess) return hip_fail(e, "MCMC --debug records");
   r = run_mcmc(m, nvox, nmeas, meas, params, cfp, fixp, nburn, njumps, sampleevery, update_proposal, rician, seed,
                voxel_offset, samples, tau_samples, m->d_state, jumps_per_launch, use_graph, s);
   if (r) return r;
@@ -756,8 +823,9 @@ int cdm_mcmc_noalloc(cdm_model* m, int nvox, int nmeas, const float* meas, float
                 " voxels per call for this model; split the volume and pass each part's voxel_offset)");
   }
   void* scratch = nullptr;
-  const hipError_t e = cdm::noalloc_scratch(&scratch);
+  hipError_t e = cdm::noalloc_scratch(&scratch);
   if (e != hipSuccess) return hip_fail(e, "allocation-free scratch lookup");
+  if ((e = debug_reset(m, 2, (hipStream_t)stream)) != hipSuccess) return hip_fail(e, "MCMC --debug records");
   return run_mcmc(m, nvox, nmeas, meas, params, cfp, fixp, nburn, njumps, sampleevery, update_proposal, rician, seed,
                   voxel_offset, samples, tau_samples, scratch, m->mcmc_block, false, (hipStream_t)stream);
 }
@@ -1037,6 +1105,7 @@ static int batch_levmar_persistent(int nitems, const cdm_batch_item* items, int 
   BatchWorkspace* w = nullptr;
   hipError_t e = batch_acquire(cdm::kBatchLevMar, host.size(), 1, &w);
   if (e == hipSuccess) e = batch_order(nitems, items, s);
+  for (int i = 0; i < nitems && e == hipSuccess; ++i) e = debug_reset(items[i].model, 1, s);
   if (e == hipSuccess) e = batch_upload(w, host, s);
   if (e == hipSuccess) e = cdm::zero_ints(w->d_queues, 1, s);
   int grid = 0;
@@ -1122,6 +1191,7 @@ static int batch_mcmc_persistent(int nitems, const cdm_batch_item* items, int nb
   BatchWorkspace* w = nullptr;
   hipError_t e = batch_acquire(cdm::kBatchMcmc, host.size(), launches.size(), &w);
   if (e == hipSuccess) e = batch_order(nitems, items, s);
+  for (int i = 0; i < nitems && e == hipSuccess; ++i) e = debug_reset(items[i].model, 2, s);
   if (e == hipSuccess) e = batch_upload(w, host, s);
   if (e != hipSuccess) return hip_fail(e, "batch MCMC set-up");
   int grid = 0;

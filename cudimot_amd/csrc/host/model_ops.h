@@ -17,6 +17,9 @@ struct Consts {
   int prior_types[kMaxParams];
   float pa[kMaxParams], pb[kMaxParams];
   int fixed[kMaxParams];
+  // --debug=<voxel>: the methods' state records of that voxel (dbg null: off)
+  int dbg_voxel, dbg_cap;
+  double* dbg;
 };
 
 struct Geom {

@@ -155,6 +155,28 @@ int cdm_mcmc_noalloc(cdm_model* m, int nvox, int nmeas, const float* meas, float
                      unsigned long long seed, long long voxel_offset, float* samples, float* tau_samples,
                      void* stream);
 
+/* --debug=<voxel> (cudimotoptions.h:160-161).  The reference prints the
+ * voxel's state from inside its kernels (GridSearch.cu:152-241,
+ * Levenberg_Marquardt.cu:482-615, MCMC.cu:437-600); here GridSearch, LM
+ * and MCMC write records of voxel `voxel` (the index within each call's
+ * volume, as the reference's index within a data part) to a device buffer
+ * of max_records records per method, which cdm_model_debug_trace copies out
+ * after the call.  voxel < 0 turns it off.  A record is 4 + nparams doubles:
+ *   GridSearch (one record): kept combination (enumeration index), its
+ *     cost, valid combinations, 0, the parameters;
+ *   LM (record 0 the start, record k after attempt k): k, cost, lambda
+ *     after the attempt, 1 if accepted, the parameters;
+ *   MCMC (record j after jump j, burn-in and recording jumps counted
+ *     together): j, energy, tau (Rician), acceptances since the last
+ *     proposal update, the parameters. */
+int cdm_model_set_debug(cdm_model* m, int voxel, int max_records);
+
+/* The records of the last call of `method` (CDM_GRIDSEARCH, CDM_LEVMAR,
+ * CDM_MCMC) into out (up to max_records of them; out may be null);
+ * *nrecords receives the number written by the call (at most the buffer's
+ * max_records).  Waits for `stream`. */
+int cdm_model_debug_trace(cdm_model* m, int method, double* out, int max_records, int* nrecords, void* stream);
+
 /* getPredictedSignal<T>::run (getPredictedSignal.cu:98-118): signal of the
  * mean sample, out [nvox][nmeas]. */
 int cdm_predicted_signal(cdm_model* m, int nvox, int nmeas, int nsamples, const float* samples, const float* cfp,

@@ -210,13 +210,17 @@ def test_split_fit_merge_equals_direct_calls(tmp_path, mcmc):
     os.makedirs(tmp_path / "out")
     opts = ["--CFP=%s/cfp.txt" % tmp_path, "--getPredictedSignal", "--BIC_AIC", "--iterLevMar=50"]
     if mcmc:
-        opts += ["--runMCMC", "--bi=40", "--nj=30", "--se=10", "--seed=77"]
+        opts += ["--runMCMC", "--bi=40", "--nj=30", "--se=10", "--seed=77", "--debug=5"]
     run("split", model, tmp_path, *common(tmp_path, nparts, opts))
     for i in range(nparts):
         a = common(tmp_path, nparts, opts)
         a[4] = "--idPart=%d" % i
         print("fit part", i, flush=True)
-        run("fit", model, tmp_path, *a)
+        r = run("fit", model, tmp_path, *a)
+        if mcmc:  # --debug=5: voxel 5 of each part, and the fit itself unchanged (below)
+            assert "----- Levenberg_Marquardt GPU algorithm: voxel 5 -----" in r.stdout, r.stdout[-2000:]
+            assert "----- MCMC GPU algorithm: voxel 5 -----" in r.stdout
+            assert r.stdout.count("Jump ") == 70
     run("merge", model, tmp_path, *common(tmp_path, nparts, opts))
 
     # the same parts through the C-ABI

@@ -105,11 +105,13 @@ def test_learned_pipeline_matches_gridsearch_pipeline():
     pg, cg = pipelines.bingham_gridsearch_lm(meas, cfp, s0)
     print("learned / GridSearch RMSE:", [round(_rmse(pl, t, k) / _rmse(pg, t, k), 4) for k in range(4)],
           "median cost ratio", np.median(host(cl)) / np.median(host(cg)), "fallback", host(fl).mean())
+    # measured: fractions 4.0% and 4.9% above GridSearch + LM, kappa and beta
+    # 0.4% and 1.5% below
     for k in (0, 1):
-        assert _rmse(pl, t, k) <= 1.15 * _rmse(pg, t, k), (k, _rmse(pl, t, k), _rmse(pg, t, k))
+        assert _rmse(pl, t, k) <= 1.08 * _rmse(pg, t, k), (k, _rmse(pl, t, k), _rmse(pg, t, k))
     for k in (2, 3):
-        assert _rmse(pl, t, k) <= 1.10 * _rmse(pg, t, k), (k, _rmse(pl, t, k), _rmse(pg, t, k))
-    assert np.median(host(cl)) <= 1.005 * np.median(host(cg))
+        assert _rmse(pl, t, k) <= 1.03 * _rmse(pg, t, k), (k, _rmse(pl, t, k), _rmse(pg, t, k))
+    assert np.median(host(cl)) <= 1.001 * np.median(host(cg))
     assert host(fl).mean() < 0.05
 
 
