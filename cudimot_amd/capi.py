@@ -75,6 +75,7 @@ def lib():
             "cdm_batch_mcmc": (i, [i, vp, i, i, i, i, i, u64, i, i, vp]),
             "cdm_mlp_create": (i, [i, i, i, vp, i, u64, ctypes.POINTER(vp)]),
             "cdm_mlp_destroy": (None, [vp]),
+            "cdm_mlp_set_train_graph": (i, [vp, i]),
             "cdm_mlp_info": (i, [vp, ip, ip, ip, ip, ctypes.POINTER(i64)]),
             "cdm_mlp_get_weights": (i, [vp, vp]),
             "cdm_mlp_set_weights": (i, [vp, vp]),
@@ -423,6 +424,11 @@ class Mlp:
         check(lib().cdm_mlp_loss_grad(self.h, n, nmeas, ptr(meas), ptr(scale), params_true.shape[1],
                                       ptr(params_true), loss.ctypes.data, grad.ctypes.data, _stream(stream)))
         return float(loss[0]), grad
+
+    def set_train_graph(self, use_graph):
+        """Training steps replayed from one captured HIP graph (default) or
+        launched one by one; identical weights either way."""
+        check(lib().cdm_mlp_set_train_graph(self.h, 1 if use_graph else 0))
 
     def train(self, clean, scale, params_true, snr=(10.0, 50.0), epochs=10, batch=4096, lr=1e-3, seed=1,
               stream=None):

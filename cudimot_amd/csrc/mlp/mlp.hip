@@ -45,6 +45,7 @@ constexpr int kMaxHead = 16;
 constexpr int kMaxLayers = 8;
 constexpr int kTile = 64;        // block tile (4 waves of 32x32)
 constexpr int kInferChunk = 65536;
+constexpr int kWgradSplits = 16;  // most K slices of a weight gradient
 
 int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
@@ -68,10 +69,14 @@ struct Epi {
   int ld_bft;
   const float* zin;   // [M][ld_zin] pre-activation (backward through gelu)
   int ld_zin;
+  size_t split_stride;  // plain, split over K: partial z goes to out_f32 + z * split_stride
 };
 
 // C[M][N] = A[M][K] B[N][K]^T with M, N multiples of 32 and K of 16; lda, ldb
-// multiples of 8.  Grid (ceil(N/64), ceil(M/64)), block 256.
+// multiples of 8.  Grid (ceil(N/64), ceil(M/64), S), block 256: with S > 1
+// (plain epilogue only) block z sums the K slice [z K, (z+1) K) into its own
+// partial C at out_f32 + z * split_stride, reduced afterwards in a fixed
+// order.
 // MFMA 32x32x16 operand layout (wave64): lane l supplies A row (l & 31) and
 // B row (l & 31), elements k0 + 8 (l >> 5) .. +7; accumulator element j of
 // lane l is C[8 (j >> 2) + 4 (l >> 5) + (j & 3)][l & 31].
@@ -83,11 +88,44 @@ __global__ void __launch_bounds__(256) gemm_nt(const bf16* __restrict__ A, int l
   const int n0 = blockIdx.x * kTile + (wave & 1) * 32;
   if (m0 >= M || n0 >= N) return;
   const int r = lane & 31, half = lane >> 5;
-  const bf16* pa = A + (size_t)(m0 + r) * lda + 8 * half;
-  const bf16* pb = B + (size_t)(n0 + r) * ldb + 8 * half;
+  const size_t kb = (size_t)blockIdx.z * K;
+  const bf16* pa = A + (size_t)(m0 + r) * lda + kb + 8 * half;
+  const bf16* pb = B + (size_t)(n0 + r) * ldb + kb + 8 * half;
   f32x16 acc = {};
-#pragma unroll 4
-  for (int k0 = 0; k0 < K; k0 += 16) {
+  // K in chunks of four MFMA steps, the next chunk's operands loaded before
+  // the current chunk's products (each step's load would otherwise wait for
+  // the memory latency alone), then the steps that remain
+  constexpr int kChunk = 4 * 16;
+  int k0 = 0;
+  if (K >= kChunk) {
+    bf16x8 a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a[u] = *(const bf16x8*)(pa + 16 * u);
+      b[u] = *(const bf16x8*)(pb + 16 * u);
+    }
+    for (; k0 + kChunk <= K; k0 += kChunk) {
+      bf16x8 an[4], bn[4];
+      const bool more = k0 + 2 * kChunk <= K;
+      if (more) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          an[u] = *(const bf16x8*)(pa + k0 + kChunk + 16 * u);
+          bn[u] = *(const bf16x8*)(pb + k0 + kChunk + 16 * u);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[u], b[u], acc, 0, 0, 0);
+      if (more) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          a[u] = an[u];
+          b[u] = bn[u];
+        }
+      }
+    }
+  }
+  for (; k0 < K; k0 += 16) {
     const bf16x8 a = *(const bf16x8*)(pa + k0);
     const bf16x8 b = *(const bf16x8*)(pb + k0);
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
@@ -103,7 +141,7 @@ __global__ void __launch_bounds__(256) gemm_nt(const bf16* __restrict__ A, int l
       const int row = row0 + i;
       float v = acc[4 * g + i];
       if (EPI == EPI_PLAIN) {
-        e.out_f32[(size_t)row * e.ld_f32 + col] = v + bias;
+        e.out_f32[blockIdx.z * e.split_stride + (size_t)row * e.ld_f32 + col] = v + bias;
       } else if (EPI == EPI_FWD_GELU) {
         const float z = v + bias;
         if (e.out_f32) e.out_f32[(size_t)row * e.ld_f32 + col] = z;
@@ -139,11 +177,25 @@ __global__ void inputs_kernel(int nrows, int rows_pad, int nmeas, int kpad, cons
 // Training inputs with fresh Rician noise: |S + s n1 + i s n2| / scale with
 // s = scale / SNR, SNR uniform in [snr_lo, snr_hi] per sample.  Draws are
 // Philox blocks addressed by (sample, measurement, epoch) under the seed.
+// Arguments of one training step that change from step to step: the batch's
+// first sample, the epoch (noise stream), the learning rate and Adam's bias
+// corrections.  A replayed graph reads them from a device table at the
+// index *ctr, which its last kernel advances; eager launches pass tab null.
+struct StepArgs {
+  long long first;
+  unsigned ep;
+  float lr, c1, c2;
+};
+
 __global__ void noisy_inputs_kernel(int rows, int nmeas, int kpad, long long first, const float* __restrict__ clean,
                                     const float* __restrict__ scale, float snr_lo, float snr_hi, uint64_t seed,
-                                    uint32_t epoch, bf16* x, bf16* xt) {
+                                    uint32_t epoch, bf16* x, bf16* xt, const StepArgs* tab, const int* ctr) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= rows * kpad) return;
+  if (tab) {
+    first = tab[*ctr].first;
+    epoch = tab[*ctr].ep;
+  }
   const int v = i / kpad, k = i - v * kpad;
   float val = 0.0f;
   if (k < nmeas) {
@@ -209,8 +261,9 @@ __global__ void encode_kernel(int n, int nparams, const float* __restrict__ para
 // zero); loss_sum accumulates the loss times loss_scale.
 __global__ void loss_kernel(int rows, int n_out, int npad, const float* __restrict__ out,
                             const float* __restrict__ target, Head h, float grad_scale, float loss_scale, bf16* dout,
-                            bf16* doutt, float* loss_sum) {
+                            bf16* doutt, float* loss_sum, const StepArgs* tab, const int* ctr) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tab) target += (size_t)tab[*ctr].first * n_out;
   float l = 0.0f;
   if (v < rows) {
     const float* o = out + (size_t)v * npad;
@@ -285,26 +338,74 @@ __global__ void decode_kernel(int n, int npad, const float* __restrict__ out, He
   }
 }
 
-// db[n] = sum_b dzt[n][b] (one wave per output unit).
-__global__ void bias_grad_kernel(int nunits, int rows, const bf16* __restrict__ dzt, float* db) {
-  const int unit = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+// One launch per layer of the weight gradient's second stage and the bias
+// gradient: blocks [0, ceil(n/256)) sum the S partial products in order,
+// gw[i] = sum_z part[z][i]; block nb + u forms db[u] = sum_b dzt[u][b]
+// (rows a multiple of 8: each thread sums eight-element vectors, then the
+// block reduces in a fixed order).
+__global__ void __launch_bounds__(256) wgrad_finish_kernel(int n, int S, const float* __restrict__ part, float* gw,
+                                                           int nunits, int rows, const bf16* __restrict__ dzt,
+                                                           float* db) {
+  const int nb = (n + 255) / 256;
+  if ((int)blockIdx.x < nb) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float acc = part[i];
+    for (int z = 1; z < S; ++z) acc += part[(size_t)z * n + i];
+    gw[i] = acc;
+    return;
+  }
+  const int unit = blockIdx.x - nb;
   if (unit >= nunits) return;
+  const bf16* row = dzt + (size_t)unit * rows;
   float s = 0.0f;
-  for (int b = lane; b < rows; b += 64) s += (float)dzt[(size_t)unit * rows + b];
+  for (int b = 8 * threadIdx.x; b < rows; b += 8 * 256) {
+    const bf16x8 q = *(const bf16x8*)(row + b);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s += (float)q[j];
+  }
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-  if (lane == 0) db[unit] = s;
+  __shared__ float red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) db[unit] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// Adam on one weight matrix [nrow][ncol] (padded storage, live region
-// [rows_live][cols_live]) or a bias vector (ncol = 1): FP32 master, moments,
-// and the BF16 copies in both layouts.
-__global__ void adam_kernel(int nrow, int ncol, int rows_live, int cols_live, float* w, const float* g, float* m,
-                            float* v, bf16* wbf, bf16* wbft, float lr, float b1, float b2, float eps, float c1,
-                            float c2) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nrow * ncol) return;
+// Adam over every weight matrix [nrow][ncol] (padded storage, live region
+// [rows_live][cols_live]) and bias vector (ncol = 1) of the network in one
+// launch: FP32 master, moments, and the BF16 copies of the matrices in both
+// layouts.  Tensor t owns the flat indices [first[t], first[t + 1]).
+struct AdamTensor {
+  int nrow, ncol, rows_live, cols_live;
+  float *w, *m, *v;
+  const float* g;
+  bf16 *wbf, *wbft;
+};
+struct AdamSet {
+  int nt;
+  int first[2 * kMaxLayers + 1];
+  AdamTensor t[2 * kMaxLayers];
+};
+
+__global__ void adam_kernel(AdamSet set, float lr, float b1, float b2, float eps, float c1, float c2,
+                            const StepArgs* tab, const int* ctr) {
+  const int gi_ = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi_ >= set.first[set.nt]) return;
+  if (tab) {
+    lr = tab[*ctr].lr;
+    c1 = tab[*ctr].c1;
+    c2 = tab[*ctr].c2;
+  }
+  int k = 0;
+  while (k + 1 < set.nt && gi_ >= set.first[k + 1]) ++k;
+  const AdamTensor& T = set.t[k];
+  const int i = gi_ - set.first[k];
+  const int nrow = T.nrow, ncol = T.ncol;
+  float *w = T.w, *m = T.m, *v = T.v;
+  const float* g = T.g;
+  bf16 *wbf = T.wbf, *wbft = T.wbft;
   const int r = i / ncol, c = i - r * ncol;
-  if (r >= rows_live || c >= cols_live) return;  // padding stays zero
+  if (r >= T.rows_live || c >= T.cols_live) return;  // padding stays zero
   const float gi = g[i];
   const float mi = b1 * m[i] + (1.0f - b1) * gi;
   const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
@@ -317,6 +418,12 @@ __global__ void adam_kernel(int nrow, int ncol, int rows_live, int cols_live, fl
     wbf[i] = h;
     wbft[(size_t)c * nrow + r] = h;
   }
+}
+
+// The last node of a training-step graph: the next replay reads the next
+// step's arguments.
+__global__ void step_advance_kernel(int* ctr) {
+  if (threadIdx.x == 0) ctr[0] = ctr[0] + 1;
 }
 
 __global__ void to_bf16_kernel(int nrow, int ncol, const float* w, bf16* wbf, bf16* wbft) {
@@ -355,6 +462,11 @@ struct cdm_mlp {
   float* loss_sum = nullptr;
   bf16 *ix = nullptr, *ih[2] = {};
   float* iout = nullptr;
+  float* wpart = nullptr;  // split-K partial weight gradients (kWgradSplits x the largest layer)
+  // training steps replayed from one captured graph (the step table and its
+  // index), or launched one by one
+  bool train_graph = true;
+  hipStream_t capture = nullptr;
 };
 
 namespace {
@@ -377,7 +489,7 @@ void free_all(cdm_mlp* m) {
   }
   for (void* p : {(void*)m->x, (void*)m->xt, (void*)m->out, (void*)m->dout, (void*)m->doutt, (void*)m->dz[0],
                   (void*)m->dz[1], (void*)m->dzt[0], (void*)m->dzt[1], (void*)m->loss_sum, (void*)m->ix,
-                  (void*)m->ih[0], (void*)m->ih[1], (void*)m->iout})
+                  (void*)m->ih[0], (void*)m->ih[1], (void*)m->iout, (void*)m->wpart})
     if (p) (void)hipFree(p);
 }
 
@@ -434,6 +546,11 @@ int ensure_train(cdm_mlp* m, int rows) {
     e = dalloc(&m->dz[i], rows * W);
     if (e == hipSuccess) e = dalloc(&m->dzt[i], rows * W);
   }
+  if (!m->wpart && e == hipSuccess) {
+    size_t most = 0;
+    for (int l = 0; l <= m->depth; ++l) most = std::max(most, (size_t)m->rows[l] * m->cols[l]);
+    e = dalloc(&m->wpart, (size_t)kWgradSplits * most);
+  }
   if (e != hipSuccess) {
     m->train_rows = 0;
     return hip_err(e, "network training workspace");
@@ -455,15 +572,42 @@ int ensure_infer(cdm_mlp* m) {
 }
 
 template <int EPI>
-void gemm(const bf16* A, int lda, const bf16* B, int ldb, int M, int N, int K, const Epi& e, hipStream_t s) {
-  hipLaunchKernelGGL(gemm_nt<EPI>, dim3((N + kTile - 1) / kTile, (M + kTile - 1) / kTile), dim3(256), 0, s, A, lda,
-                     B, ldb, M, N, K, e);
+void gemm(const bf16* A, int lda, const bf16* B, int ldb, int M, int N, int K, const Epi& e, hipStream_t s,
+          int splits = 1) {
+  hipLaunchKernelGGL(gemm_nt<EPI>, dim3((N + kTile - 1) / kTile, (M + kTile - 1) / kTile, splits), dim3(256), 0, s, A,
+                     lda, B, ldb, M, N, K / splits, e);
 }
+
+
 
 Epi epi0() {
   Epi e;
   std::memset(&e, 0, sizeof(e));
   return e;
+}
+
+// K slices of a weight gradient (K = the batch rows): a 256 x 256 layer is
+// only 16 output tiles, so without slices 16 blocks would each run the whole
+// batch.  The slices keep K a multiple of 16 and at least 64 rows each.
+int wgrad_splits(int rows) {
+  int sp = kWgradSplits;
+  while (sp > 1 && (rows % (16 * sp) != 0 || rows / sp < 64)) sp /= 2;
+  return sp;
+}
+
+// dW = dzt h^T as K slices into m->wpart, then their ordered sum and the bias
+// gradient (sum over the batch of dzt) in one launch.
+void wgrad(cdm_mlp* m, const bf16* dzt, int units, const bf16* hin_t, int cols, int rows, float* gw, float* gb,
+           hipStream_t s) {
+  const int sp = wgrad_splits(rows);
+  Epi g = epi0();
+  g.out_f32 = m->wpart;
+  g.ld_f32 = cols;
+  g.split_stride = (size_t)units * cols;
+  gemm<EPI_PLAIN>(dzt, rows, hin_t, rows, units, cols, rows, g, s, sp);
+  const int n = units * cols;
+  hipLaunchKernelGGL(wgrad_finish_kernel, dim3((n + 255) / 256 + units), dim3(256), 0, s, n, sp, m->wpart, gw, units,
+                     rows, dzt, gb);
 }
 
 // Forward over `rows` (multiple of 32) rows of x.  Training keeps z, h, ht.
@@ -494,13 +638,7 @@ void forward_train(cdm_mlp* m, int rows, hipStream_t s) {
 void backward(cdm_mlp* m, int rows, hipStream_t s) {
   const int L = m->depth;
   // output layer
-  {
-    Epi e = epi0();
-    e.out_f32 = m->gw[L];
-    e.ld_f32 = m->width;
-    gemm<EPI_PLAIN>(m->doutt, rows, m->ht[L - 1], rows, m->npad, m->width, rows, e, s);
-    hipLaunchKernelGGL(bias_grad_kernel, dim3((m->npad + 3) / 4), dim3(256), 0, s, m->npad, rows, m->doutt, m->gb[L]);
-  }
+  wgrad(m, m->doutt, m->npad, m->ht[L - 1], m->width, rows, m->gw[L], m->gb[L], s);
   // dH_{L-1} = dout W_L, through gelu' -> dz
   const bf16* dcur = m->dout;
   int ld_dcur = m->npad, kcur = m->npad;
@@ -516,31 +654,42 @@ void backward(cdm_mlp* m, int rows, hipStream_t s) {
     e.ld_bft = rows;
     gemm<EPI_BWD_GELU>(dcur, ld_dcur, wt, kcur, rows, m->width, kcur, e, s);
     // dW_l = dz^T h_{l-1}
-    Epi g = epi0();
-    g.out_f32 = m->gw[l];
-    g.ld_f32 = m->cols[l];
     const bf16* hin_t = l > 0 ? m->ht[l - 1] : m->xt;
-    gemm<EPI_PLAIN>(m->dzt[slot], rows, hin_t, rows, m->width, m->cols[l], rows, g, s);
-    hipLaunchKernelGGL(bias_grad_kernel, dim3((m->width + 3) / 4), dim3(256), 0, s, m->width, rows, m->dzt[slot],
-                       m->gb[l]);
+    wgrad(m, m->dzt[slot], m->width, hin_t, m->cols[l], rows, m->gw[l], m->gb[l], s);
     dcur = m->dz[slot];
     ld_dcur = kcur = m->width;
     wt = m->wbft[l];
   }
 }
 
-void adam(cdm_mlp* m, float lr, hipStream_t s) {
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+constexpr float kAdamB1 = 0.9f, kAdamB2 = 0.999f, kAdamEps = 1e-8f;
+
+// Adam's bias corrections at step t (1-based).
+float adam_c1(long long t) { return 1.0f - std::pow(kAdamB1, (float)t); }
+float adam_c2(long long t) { return 1.0f - std::pow(kAdamB2, (float)t); }
+
+// One Adam step over every tensor; with tab (a replayed graph) lr and the
+// corrections come from the step table instead of the arguments.
+void adam(cdm_mlp* m, float lr, hipStream_t s, const StepArgs* tab = nullptr, const int* ctr = nullptr) {
   ++m->adam_step;
-  const float c1 = 1.0f - std::pow(b1, (float)m->adam_step), c2 = 1.0f - std::pow(b2, (float)m->adam_step);
+  const float c1 = adam_c1(m->adam_step), c2 = adam_c2(m->adam_step);
+  AdamSet set;
+  std::memset(&set, 0, sizeof(set));
+  int total = 0;
   for (int l = 0; l <= m->depth; ++l) {
-    const int n = m->rows[l] * m->cols[l];
-    hipLaunchKernelGGL(adam_kernel, grid1(n), dim3(256), 0, s, m->rows[l], m->cols[l], m->rows_live[l],
-                       m->cols_live[l], m->w[l], m->gw[l], m->mw[l], m->vw[l], m->wbf[l], m->wbft[l], lr, b1, b2, eps,
-                       c1, c2);
-    hipLaunchKernelGGL(adam_kernel, grid1(m->rows[l]), dim3(256), 0, s, m->rows[l], 1, m->rows_live[l], 1, m->b[l],
-                       m->gb[l], m->mb[l], m->vb[l], (bf16*)nullptr, (bf16*)nullptr, lr, b1, b2, eps, c1, c2);
+    AdamTensor& tw = set.t[set.nt];
+    tw = {m->rows[l], m->cols[l], m->rows_live[l], m->cols_live[l], m->w[l], m->mw[l], m->vw[l], m->gw[l],
+          m->wbf[l], m->wbft[l]};
+    set.first[set.nt++] = total;
+    total += m->rows[l] * m->cols[l];
+    AdamTensor& tb = set.t[set.nt];
+    tb = {m->rows[l], 1, m->rows_live[l], 1, m->b[l], m->mb[l], m->vb[l], m->gb[l], nullptr, nullptr};
+    set.first[set.nt++] = total;
+    total += m->rows[l];
   }
+  set.first[set.nt] = total;
+  hipLaunchKernelGGL(adam_kernel, grid1(total), dim3(256), 0, s, set, lr, kAdamB1, kAdamB2, kAdamEps, c1, c2, tab,
+                     ctr);
 }
 
 // Inference over n voxels in chunks: raw outputs to out [n][npad] (device,
@@ -673,7 +822,14 @@ void cdm_mlp_destroy(cdm_mlp* m) {
   if (!m) return;
   (void)hipDeviceSynchronize();
   free_all(m);
+  if (m->capture) (void)hipStreamDestroy(m->capture);
   delete m;
+}
+
+int cdm_mlp_set_train_graph(cdm_mlp* m, int use_graph) {
+  if (!m) return cdm::set_error("null network handle");
+  m->train_graph = use_graph != 0;
+  return 0;
 }
 
 int cdm_mlp_info(const cdm_mlp* m, int* n_in, int* width, int* depth, int* n_out, long long* nweights) {
@@ -768,7 +924,7 @@ int cdm_mlp_loss_grad(cdm_mlp* m, int n, int nmeas, const float* meas, const flo
   forward_train(m, n, s);
   (void)hipMemsetAsync(m->loss_sum, 0, sizeof(float), s);
   hipLaunchKernelGGL(loss_kernel, grid1(n), dim3(256), 0, s, n, m->n_out, m->npad, m->out, target, m->head,
-                     1.0f / n, 1.0f / n, m->dout, m->doutt, m->loss_sum);
+                     1.0f / n, 1.0f / n, m->dout, m->doutt, m->loss_sum, (const StepArgs*)nullptr, (const int*)nullptr);
   backward(m, n, s);
   e = hipStreamSynchronize(s);
   if (e == hipSuccess) e = hipGetLastError();
@@ -806,26 +962,75 @@ int cdm_mlp_train(cdm_mlp* m, int n, int nmeas, const float* clean, const float*
   hipLaunchKernelGGL(encode_kernel, grid1(n), dim3(256), 0, s, n, nparams, params_true, m->head, m->n_out, target);
   const int nb = n / batch;  // whole batches; the remainder rotates in through the epoch offset
   const long long total = (long long)nb * epochs;
+  // every step's changing arguments, as the eager loop computes them
+  std::vector<StepArgs> steps((size_t)total);
+  for (long long st = 0; st < total; ++st) {
+    const int ep = (int)(st / nb), bi = (int)(st % nb);
+    const long long shift = (long long)(ep * 7919LL % std::max(1, n - nb * batch + 1));
+    const float cosw = 0.5f * (1.0f + std::cos(3.14159265f * (float)st / (float)std::max(1LL, total)));
+    StepArgs& a = steps[(size_t)st];
+    a.first = shift + (long long)bi * batch;
+    a.ep = (unsigned)ep;
+    a.lr = lr * (0.02f + 0.98f * cosw);
+    a.c1 = adam_c1(m->adam_step + st + 1);
+    a.c2 = adam_c2(m->adam_step + st + 1);
+  }
+  auto step_kernels = [&](hipStream_t q, const StepArgs& a, const StepArgs* tab, const int* ctr) {
+    hipLaunchKernelGGL(noisy_inputs_kernel, grid1((long long)batch * m->kpad), dim3(256), 0, q, batch, nmeas,
+                       m->kpad, a.first, clean, scale, snr_lo, snr_hi, (uint64_t)seed, (uint32_t)a.ep, m->x, m->xt,
+                       tab, ctr);
+    forward_train(m, batch, q);
+    hipLaunchKernelGGL(loss_kernel, grid1(batch), dim3(256), 0, q, batch, m->n_out, m->npad, m->out,
+                       tab ? target : target + (size_t)a.first * m->n_out, m->head, 1.0f / batch,
+                       1.0f / (batch * (float)nb), m->dout, m->doutt, m->loss_sum, tab, ctr);
+    backward(m, batch, q);
+    adam(m, a.lr, q, tab, ctr);
+  };
+  // Graph: one step captured with its changing arguments read from the
+  // device step table at *ctr, replayed once per step; its last node
+  // advances *ctr.  The table, the index and the per-epoch loss reset are
+  // written outside the graph.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  StepArgs* d_tab = nullptr;
+  int* d_ctr = nullptr;
+  if (m->train_graph && total > 0) {
+    e = dalloc(&d_tab, (size_t)total);
+    if (e == hipSuccess) e = dalloc(&d_ctr, 1);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_tab, steps.data(), steps.size() * sizeof(StepArgs), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemsetAsync(d_ctr, 0, sizeof(int), s);
+    if (e == hipSuccess && !m->capture) e = hipStreamCreateWithFlags(&m->capture, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(m->capture, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      const long long step0 = m->adam_step;
+      step_kernels(m->capture, steps[0], d_tab, d_ctr);
+      m->adam_step = step0;  // the replays advance it below
+      hipLaunchKernelGGL(step_advance_kernel, dim3(1), dim3(64), 0, m->capture, d_ctr);
+      const hipError_t ec = hipStreamEndCapture(m->capture, &graph);
+      e = e == hipSuccess ? ec : e;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  }
   long long step = 0;
   for (int ep = 0; ep < epochs && e == hipSuccess; ++ep) {
-    (void)hipMemsetAsync(m->loss_sum, 0, sizeof(float), s);
-    const long long shift = (long long)(ep * 7919LL % std::max(1, n - nb * batch + 1));
-    for (int bi = 0; bi < nb; ++bi, ++step) {
-      const long long first = shift + (long long)bi * batch;
-      hipLaunchKernelGGL(noisy_inputs_kernel, grid1((long long)batch * m->kpad), dim3(256), 0, s, batch, nmeas,
-                         m->kpad, first, clean, scale, snr_lo, snr_hi, (uint64_t)seed, (uint32_t)ep, m->x, m->xt);
-      forward_train(m, batch, s);
-      hipLaunchKernelGGL(loss_kernel, grid1(batch), dim3(256), 0, s, batch, m->n_out, m->npad, m->out,
-                         target + (size_t)first * m->n_out, m->head, 1.0f / batch, 1.0f / (batch * (float)nb),
-                         m->dout, m->doutt, m->loss_sum);
-      backward(m, batch, s);
-      const float cosw = 0.5f * (1.0f + std::cos(3.14159265f * (float)step / (float)std::max(1LL, total)));
-      adam(m, lr * (0.02f + 0.98f * cosw), s);
+    e = hipMemsetAsync(m->loss_sum, 0, sizeof(float), s);
+    for (int bi = 0; bi < nb && e == hipSuccess; ++bi, ++step) {
+      if (exec) {
+        e = hipGraphLaunch(exec, s);
+        ++m->adam_step;
+      } else {
+        step_kernels(s, steps[(size_t)step], nullptr, nullptr);
+      }
     }
-    e = hipGetLastError();
+    if (e == hipSuccess) e = hipGetLastError();
     if (e == hipSuccess && epoch_loss) e = hipMemcpyAsync(&epoch_loss[ep], m->loss_sum, 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
   }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipStreamSynchronize(s);
+  if (d_tab) (void)hipFree(d_tab);
+  if (d_ctr) (void)hipFree(d_ctr);
   (void)hipStreamSynchronize(s);
   (void)hipFree(target);
   return e == hipSuccess ? 0 : hip_err(e, "network training");

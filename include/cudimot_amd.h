@@ -256,6 +256,13 @@ int cdm_mlp_train(cdm_mlp* m, int n, int nmeas, const float* clean, const float*
                   const float* params_true, float snr_lo, float snr_hi, int epochs, int batch, float lr,
                   unsigned long long seed, float* epoch_loss, void* stream);
 
+/* use_graph != 0 (the default): cdm_mlp_train records one training step into
+ * a HIP graph and replays it once per step, the step's changing arguments
+ * (batch offset, noise epoch, learning rate, Adam's bias corrections) read
+ * from a device table; 0: every step's kernels are launched one by one.  The
+ * same kernels and arguments either way: the weights are identical. */
+int cdm_mlp_set_train_graph(cdm_mlp* m, int use_graph);
+
 /* ------------------------------------------------------------------------
  * Batched persistent launch.  No counterpart in the reference, which runs one
  * binary per model over its voxel sub-parts in turn (cudimot.cc:81-120).

@@ -202,3 +202,24 @@ def test_mlp_training_lowers_loss_and_beats_default_start():
         capi.sync()
         cost[name] = host(c)
     assert np.median(cost["net"]) < np.median(cost["default"]), (np.median(cost["net"]), np.median(cost["default"]))
+
+
+@pytest.mark.gpu
+def test_mlp_training_graph_equals_step_by_step_launches():
+    """cdm_mlp_train replays one captured step graph (its changing arguments
+    from a device table) or launches every step's kernels: the same weights
+    after three epochs, and training twice in a row (the graph is captured
+    anew, the Adam step count carried on) also agrees."""
+    from cudimot_amd import capi
+    t, clean, _ = bingham_signals(4096, 41)
+    weights = {}
+    for use_graph in (True, False):
+        net = make_net(seed=22)
+        net.set_train_graph(use_graph)
+        for rep in range(2):
+            net.train(dev(clean), None, dev(t), snr=(10, 50), epochs=3, batch=512, lr=2e-3, seed=5 + rep)
+            capi.sync()
+            weights[(use_graph, rep)] = net.get_weights()
+    for rep in range(2):
+        np.testing.assert_array_equal(weights[(True, rep)], weights[(False, rep)])
+    assert not np.array_equal(weights[(True, 0)], weights[(True, 1)])
