@@ -145,8 +145,11 @@ def batch_unit_source(name, model_dirs):
             "  return mode == cdm::kBatchLevMar ? batch_kernel<cdm::kBatchLevMar>",
             "         : mode == cdm::kBatchMcmc  ? batch_kernel<cdm::kBatchMcmc>",
             "                                    : batch_kernel<cdm::kBatchMcmcRician>;", "}", ""]
-    out += ["static int prepare(int mode, int nitems, const cdm::BatchLaunchItem* in, void* host_out, int* total_tiles,",
-            "                   size_t* lds_bytes) {",
+    out += ["// the persistent MCMC kernel keeps cache rows in LDS up to 80 KB a block, as a",
+            "// two-blocks-per-CU budget would",
+            "constexpr int kBatchMcmcBudget = 2;",
+            "static int prepare(int mode, int nitems, const cdm::BatchLaunchItem* in, void* host_out, int* total_tiles,"]
+    out += ["                   size_t* lds_bytes) {",
             "  DevItem* out = static_cast<DevItem*>(host_out);", "  int tiles = 0;", "  size_t lds = 0;",
             "  for (int i = 0; i < nitems; ++i) {", "    const cdm::BatchLaunchItem& b = in[i];",
             "    if (b.model < 0 || b.model >= %d || b.g.nvox <= 0) return -1;" % len(model_dirs),
@@ -157,12 +160,12 @@ def batch_unit_source(name, model_dirs):
                 "        if (b.g.cfp_ts > %s::cdm_cfp_max || b.g.fixp_ts > %s::cdm_fixp_max) return -1;" % (ns(k), ns(k)),
                 "        // LM groups of 64 lanes past the LDS fall back to 16, as the per-model launch",
                 "        if (mode == cdm::kBatchLevMar) L = %s::op_lm_lanes(b.g, L);" % ns(k),
-                "        need = mode != cdm::kBatchLevMar ? %s::op_lds_mcmc(b.g, L) : %s::op_lds_lm(b.g, L);" % (ns(k), ns(k)),
+                "        need = mode != cdm::kBatchLevMar ? %s::op_lds_mcmc(b.g, L, kBatchMcmcBudget) : %s::op_lds_lm(b.g, L);" % (ns(k), ns(k)),
                 "        if (need > %s::cdm_lds_max) return -1;" % ns(k),
                 "        if (mode == cdm::kBatchLevMar) {",
                 "          const auto a = %s::op_lm_args(b.g, b.params, b.max_iters, b.c, b.iters, b.cost);" % ns(k),
                 "          std::memcpy(d.args, &a, sizeof(a));", "        } else {",
-                "          const auto a = %s::op_mcmc_args(b.g, L, b.mc, b.c);" % ns(k),
+                "          const auto a = %s::op_mcmc_args(b.g, L, kBatchMcmcBudget, b.mc, b.c);" % ns(k),
                 "          std::memcpy(d.args, &a, sizeof(a));", "        }", "      } break;"]
     out += ["      default: return -1;", "    }",
             "    const int li = L == 1 ? 0 : L == 4 ? 1 : L == 16 ? 2 : L == 64 ? 3 : -1;",

@@ -32,19 +32,21 @@ static cdm_geom op_geom(const cdm::Geom& g) {
   return k;
 }
 
-// MCMC keeps the per-measurement cache rows in LDS when the block's LDS then
-// stays within this bound (two blocks per CU), else in global memory.
-constexpr size_t cdm_mcmc_lds_rows_max = 80 * 1024;
+// LDS of a CU (gfx950), the most one block can have.
+constexpr size_t cdm_lds_max = 160 * 1024;
 
-static size_t op_mcmc_rows_lds_bytes(const cdm::Geom& g, int L) {
+// MCMC keeps the per-measurement cache rows in LDS when the block's LDS then
+// leaves room for the W blocks per CU of the launch's register budget (W = 1:
+// the whole LDS of the CU; 2: 80 KB), else in global memory.
+static size_t op_mcmc_rows_lds_bytes(const cdm::Geom& g, int L, int W) {
   if (cdm_mvc <= 0) return 0;
   const size_t rows = (size_t)(cdm_block / L) * 2 * (size_t)cdm_mcmc_row_stride(g.nmeas) * sizeof(cdm_tf);
   const size_t total = cdm_mcmc_rows_offset(g.nmeas, g.cfp_ts, L) + rows;
-  return total <= cdm_mcmc_lds_rows_max ? rows : 0;
+  return total <= cdm_lds_max / (size_t)(W > 1 ? W : 1) ? rows : 0;
 }
 
-static size_t op_lds_mcmc(const cdm::Geom& g, int L) {
-  const size_t rows = op_mcmc_rows_lds_bytes(g, L);
+static size_t op_lds_mcmc(const cdm::Geom& g, int L, int W) {
+  const size_t rows = op_mcmc_rows_lds_bytes(g, L, W);
   if (rows) return cdm_mcmc_rows_offset(g.nmeas, g.cfp_ts, L) + rows;
   return cdm_vslot_offset(g.nmeas, g.cfp_ts) + (size_t)(cdm_block / L) * sizeof(cdm_vslot);
 }
@@ -52,9 +54,6 @@ static size_t op_lds_mcmc(const cdm::Geom& g, int L) {
 static size_t op_lds(const cdm::Geom& g) {
   return (size_t)cdm_mc_offset(g.nmeas, g.cfp_ts) * sizeof(float) + (size_t)g.nmeas * cdm_mc * sizeof(CDM_F64);
 }
-
-// LDS of a CU (gfx950), the most one block can have.
-constexpr size_t cdm_lds_max = 160 * 1024;
 
 // LM launches: the 64-lane groups' J^T J scratch after the shared tables.
 static size_t op_lds_lm(const cdm::Geom& g, int L) {
@@ -88,13 +87,15 @@ static cdm_lm_args op_lm_args(const cdm::Geom& g, float* params, int max_iters, 
   return a;
 }
 
-// L: the launch's voxel-group size (it sets the LDS layout of the rows).
-static cdm_mcmc_args op_mcmc_args(const cdm::Geom& g, int L, const cdm::McmcLaunch& m, const cdm::Consts& c) {
+// L, W: the launch's voxel-group size and register budget (they decide where
+// the cache rows live and their LDS layout).
+static cdm_mcmc_args op_mcmc_args(const cdm::Geom& g, int L, int W, const cdm::McmcLaunch& m,
+                                  const cdm::Consts& c) {
   cdm_mcmc_args a;
   a.g = op_geom(g);
   a.params = m.params;
   a.mvc = (cdm_tf*)m.mvc;
-  a.rows_lds = op_mcmc_rows_lds_bytes(g, L) > 0 ? 1 : 0;
+  a.rows_lds = op_mcmc_rows_lds_bytes(g, L, W) > 0 ? 1 : 0;
   a.row_stride = cdm_mcmc_row_stride(g.nmeas);
   a.state = (cdm_chain*)m.state;
   a.samples = m.samples;

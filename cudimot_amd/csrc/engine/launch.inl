@@ -92,18 +92,23 @@ static hipError_t op_mcmc(int L, int W, int rician, const cdm::Geom& g, const cd
                           const cdm::Consts& c, hipStream_t s) {
   if (g.nvox <= 0) return hipSuccess;
   if (g.cfp_ts > cdm_cfp_max || g.fixp_ts > cdm_fixp_max) return hipErrorInvalidValue;
-  const cdm_mcmc_args a = op_mcmc_args(g, L, m, c);
-  if (rician) {  // the Rician likelihood is compiled at the default register budget only
+  // the Rician likelihood is compiled at the default register budget only
+  const int We = rician ? 1 : W;
+  const cdm_mcmc_args a = op_mcmc_args(g, L, We, m, c);
+  if (rician) {
     CDM_DISPATCH_L(L, hipLaunchKernelGGL((cdm_mcmc_kernel<LG, true, 1>), op_grid(g.nvox, LG), dim3(cdm_block),
-                                         op_lds_mcmc(g, LG), s, a));
+                                         op_lds_mcmc(g, LG, We), s, a));
   } else {
     CDM_DISPATCH_L(L, CDM_DISPATCH_W(W, hipLaunchKernelGGL((cdm_mcmc_kernel<LG, false, WG>), op_grid(g.nvox, LG),
-                                                           dim3(cdm_block), op_lds_mcmc(g, LG), s, a)));
+                                                           dim3(cdm_block), op_lds_mcmc(g, LG, We), s, a)));
   }
   return hipGetLastError();
 }
 
-static int op_mcmc_rows_lds(int L, const cdm::Geom& g) { return op_mcmc_rows_lds_bytes(g, L) > 0 ? 1 : 0; }
+// W: the launch's effective register budget (1 for the Rician likelihood).
+static int op_mcmc_rows_lds(int L, int W, const cdm::Geom& g) {
+  return op_mcmc_rows_lds_bytes(g, L, W) > 0 ? 1 : 0;
+}
 
 static hipError_t op_predicted(int L, const cdm::Geom& g, int nsamples, const float* samples, float* out,
                                hipStream_t s) {

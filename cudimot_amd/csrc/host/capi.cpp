@@ -703,12 +703,18 @@ int check_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, const float
   return 0;
 }
 
+// Register budget an MCMC launch runs at (the Rician likelihood is compiled
+// at the default budget only); with the voxel group it decides where the
+// per-measurement cache rows live.
+int mcmc_budget(const cdm_model* m, int nmeas, int rician) { return rician ? 1 : occ_for(m, cdm::kMcmc, nmeas); }
+
 // Bytes of chain state an MCMC call over nvox voxels needs: the chains, then
 // the per-measurement cache rows unless the launch keeps them in LDS.
-size_t mcmc_state_bytes(const cdm_model* m, int nvox, int nmeas, const float* cfp, const float* fixp) {
+size_t mcmc_state_bytes(const cdm_model* m, int nvox, int nmeas, const float* cfp, const float* fixp, int rician) {
   const int L = group_for(m, cdm::kMcmc, nmeas, nvox);
   const cdm::Geom g = make_geom(m, nvox, nmeas, nullptr, cfp, fixp);
-  if (!m->ops->mvc_bytes_per_meas || m->ops->mcmc_rows_lds(L, g)) return m->ops->chain_bytes * (size_t)nvox;
+  if (!m->ops->mvc_bytes_per_meas || m->ops->mcmc_rows_lds(L, mcmc_budget(m, nmeas, rician), g))
+    return m->ops->chain_bytes * (size_t)nvox;
   return mvc_offset(m, nvox) + m->ops->mvc_bytes_per_meas * 2 * (size_t)nmeas * (size_t)nvox;
 }
 
@@ -730,8 +736,9 @@ int run_mcmc(cdm_model* m, int nvox, int nmeas, const float* meas, float* params
   cdm::McmcLaunch ml;
   ml.params = params;
   ml.state = state;
-  ml.mvc = (m->ops->mvc_bytes_per_meas && !m->ops->mcmc_rows_lds(L, g)) ? (void*)((char*)state + mvc_offset(m, nvox))
-                                                                       : nullptr;
+  ml.mvc = (m->ops->mvc_bytes_per_meas && !m->ops->mcmc_rows_lds(L, mcmc_budget(m, nmeas, rician), g))
+               ? (void*)((char*)state + mvc_offset(m, nvox))
+               : nullptr;
   ml.samples = samples;
   ml.tau_samples = tau_samples;
   ml.nsamples = nsamples;
@@ -782,7 +789,7 @@ int mcmc_call(cdm_model* m, int nvox, int nmeas, const float* meas, float* param
               int nburn, int njumps, int sampleevery, int update_proposal, int rician, unsigned long long seed,
               long long voxel_offset, float* samples, float* tau_samples, int jumps_per_launch, bool use_graph,
               hipStream_t s) {
-  int r = ensure_state(m, mcmc_state_bytes(m, nvox, nmeas, cfp, fixp));
+  int r = ensure_state(m, mcmc_state_bytes(m, nvox, nmeas, cfp, fixp, rician));
   if (r) return r;
   hipError_t e = order_after_last(m, s);
   if (e != hipSuccess) return hip_fail(e, "MCMC stream order");
@@ -813,7 +820,7 @@ int cdm_mcmc_noalloc(cdm_model* m, int nvox, int nmeas, const float* meas, float
                      void* stream) {
   int r = check_mcmc(m, nvox, nmeas, meas, cfp, fixp, njumps, sampleevery, rician, samples, tau_samples);
   if (r || nvox == 0) return r;
-  const size_t need = mcmc_state_bytes(m, nvox, nmeas, cfp, fixp);
+  const size_t need = mcmc_state_bytes(m, nvox, nmeas, cfp, fixp, rician);
   if (need > cdm::kNoAllocScratchBytes) {
     const size_t per = (need + (size_t)nvox - 1) / (size_t)nvox;
     return fail("cdm_mcmc_noalloc: " + std::to_string(nvox) + " voxels x " + std::to_string(nmeas) +

@@ -70,7 +70,7 @@ struct ModelOps {
                         const float* tau_samples, float* bic, float* aic, hipStream_t s);
   // 1 when an MCMC launch of group size L keeps the per-measurement cache
   // rows in LDS (the state buffer then needs no rows after the chains)
-  int (*mcmc_rows_lds)(int L, const Geom& g);
+  int (*mcmc_rows_lds)(int L, int W, const Geom& g);
 };
 
 // Voxel-group sizes compiled for every model and method.
