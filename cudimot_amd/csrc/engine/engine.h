@@ -98,6 +98,14 @@ constexpr unsigned cdm_mvc_dep = CDM_MEAS_VOXEL_CACHE_DEPENDS;
 constexpr int cdm_mvc = 0;
 constexpr unsigned cdm_mvc_dep = 0;
 #endif
+// The column-wise LM Jacobian keeps the base point's per-measurement cache
+// values in LDS, so that a probe recomputes only the entries its parameter
+// moves (Predicted_Signal_MV_Update).
+#if defined(CDM_LM_JACOBIAN_COLUMNS) && defined(CDM_MEAS_VOXEL_CACHE_PARTIAL)
+constexpr int cdm_lm_base_mvc = CDM_MEAS_VOXEL_CACHE;
+#else
+constexpr int cdm_lm_base_mvc = 0;
+#endif
 
 template <class TF>
 __device__ __forceinline__ void cdm_prepare(TF* p, TF* fx, CDM_F64* vc) {
@@ -769,9 +777,11 @@ __host__ __device__ __forceinline__ size_t cdm_lm_scratch_offset(int nmeas, int 
 // the P(P+1)/2 + P sums (J^T J packed, then J^T r; J^T J is kept there,
 // doubled and with the fixed parameters' rows and columns replaced, for the
 // damped solves of rejected steps), then the P(P+1)/2 entries of the
-// factorised damped matrix; none below 64 lanes.
+// factorised damped matrix, then (column-wise Jacobian with partial
+// per-measurement caches) the base point's [M][cdm_lm_base_mvc] cache values;
+// none below 64 lanes.
 __host__ __device__ __forceinline__ size_t cdm_lm_scratch_doubles(int L, int nmeas) {
-  return L == 64 ? (size_t)nmeas * (cdm_np + 1) + 2 * cdm_ntri + cdm_np : 0;
+  return L == 64 ? (size_t)nmeas * (cdm_np + 1) + 2 * cdm_ntri + cdm_np + (size_t)nmeas * cdm_lm_base_mvc : 0;
 }
 
 // LDS stores of one wave made visible to the wave's other lanes.
@@ -825,13 +835,22 @@ __device__ __forceinline__ void cdm_lm_jacobian_columns(const cdm_geom& g, const
   CDM_F64 jr0[cdm_vc_jobs], vc0[cdm_vc];
   cdm_cache_jobs_g<L>(pj, fxj, jr0, lane);
   Voxel_Cache_Combine(cdm_np, pj, fxj, jr0, vc0);
+  // the base point's per-measurement cache values, after the factor storage
+  CDM_F64* base_mvc = rows + (size_t)g.nmeas * np1 + 2 * cdm_ntri + cdm_np;
+  (void)base_mvc;
   // signal, residual and analytic entries
   for (int m = lane; m < g.nmeas; m += L) {
     cdm_jt* cm = rowj.at(s_cfp, m, g.cfp_ts);
     cdm_jt d[cdm_np];
 #pragma unroll
     for (int i = 0; i < cdm_np; ++i) d[i] = (cdm_jt)0;
-#ifdef CDM_SIGNAL_PARTIALS
+#if defined(CDM_MEAS_VOXEL_CACHE_PARTIAL)
+    cdm_jt mvc[cdm_mvc];
+    const cdm_jt sig = Predicted_Signal_MV_Update(cdm_np, pj, cm, fxj, vc0, cdm_mc_row(s_mc, m), mvc, mvc, -1);
+#pragma unroll
+    for (int k = 0; k < cdm_mvc; ++k) base_mvc[m * cdm_mvc + k] = (CDM_F64)mvc[k];
+    Partial_Derivatives_Cached(cdm_np, pj, cm, fxj, vc0, cdm_mc_row(s_mc, m), d);
+#elif defined(CDM_SIGNAL_PARTIALS)
     const cdm_jt sig = Predicted_Signal_Partials_Cached(cdm_np, pj, cm, fxj, vc0, cdm_mc_row(s_mc, m), d);
 #else
     const cdm_jt sig = cdm_signal(pj, cm, fxj, vc0, cdm_mc_row(s_mc, m));
@@ -895,7 +914,16 @@ __device__ __forceinline__ void cdm_lm_jacobian_columns(const cdm_geom& g, const
       }
       for (int m = lane; m < g.nmeas; m += L) {
         cdm_jt* cm = rowj.at(s_cfp, m, g.cfp_ts);
+#if defined(CDM_MEAS_VOXEL_CACHE_PARTIAL)
+        // only the cache entries parameter i moves are recomputed; the rest
+        // are the base point's (the same values a full evaluation gives)
+        cdm_jt cur[cdm_mvc], tmp[cdm_mvc];
+#pragma unroll
+        for (int k = 0; k < cdm_mvc; ++k) cur[k] = (cdm_jt)base_mvc[m * cdm_mvc + k];
+        const cdm_jt sq = Predicted_Signal_MV_Update(cdm_np, q, cm, fxj, vc, cdm_mc_row(s_mc, m), cur, tmp, i);
+#else
         const cdm_jt sq = cdm_signal(q, cm, fxj, vc, cdm_mc_row(s_mc, m));
+#endif
         if (probe == 0) {
           rows[m * np1 + i] = (CDM_F64)sq;
         } else {
