@@ -773,15 +773,32 @@ __host__ __device__ __forceinline__ size_t cdm_lm_scratch_offset(int nmeas, int 
   return (b + 15) / 16 * 16;
 }
 
+// The column-wise LM Jacobian builds the voxel caches of all its
+// central-difference probes at once, one probe per lane, into LDS (after the
+// base point's per-measurement cache values): the jobs of the probes whose
+// parameter moves them, [2 x those parameters][jobs], then the caches of
+// the probes whose parameter moves the cache, [2 x those parameters][cache].
+#if defined(CDM_LM_JACOBIAN_COLUMNS) && defined(CDM_NUMERICAL_MASK) && defined(CDM_VOXEL_CACHE) && \
+    defined(CDM_VOXEL_CACHE_JOBS)
+constexpr unsigned cdm_lm_kdep = CDM_NUMERICAL_MASK & cdm_vc_dep;
+constexpr unsigned cdm_lm_kjdep = CDM_NUMERICAL_MASK & cdm_vcj_dep;
+constexpr size_t cdm_lm_probe_doubles =
+    (size_t)2 * __builtin_popcount(cdm_lm_kjdep) * cdm_vc_jobs + (size_t)2 * __builtin_popcount(cdm_lm_kdep) * cdm_vc;
+#else
+constexpr size_t cdm_lm_probe_doubles = 0;
+#endif
+
 // Doubles of LDS scratch per LM voxel group: the [M][P+1] rows (J, r), then
 // the P(P+1)/2 + P sums (J^T J packed, then J^T r; J^T J is kept there,
 // doubled and with the fixed parameters' rows and columns replaced, for the
 // damped solves of rejected steps), then the P(P+1)/2 entries of the
 // factorised damped matrix, then (column-wise Jacobian with partial
-// per-measurement caches) the base point's [M][cdm_lm_base_mvc] cache values;
-// none below 64 lanes.
+// per-measurement caches) the base point's [M][cdm_lm_base_mvc] cache values,
+// then the column-wise Jacobian's probe jobs and caches; none below 64 lanes.
 __host__ __device__ __forceinline__ size_t cdm_lm_scratch_doubles(int L, int nmeas) {
-  return L == 64 ? (size_t)nmeas * (cdm_np + 1) + 2 * cdm_ntri + cdm_np + (size_t)nmeas * cdm_lm_base_mvc : 0;
+  return L == 64 ? (size_t)nmeas * (cdm_np + 1) + 2 * cdm_ntri + cdm_np + (size_t)nmeas * cdm_lm_base_mvc +
+                       cdm_lm_probe_doubles
+                 : 0;
 }
 
 // LDS stores of one wave made visible to the wave's other lanes.
@@ -861,6 +878,73 @@ __device__ __forceinline__ void cdm_lm_jacobian_columns(const cdm_geom& g, const
       if (!((CDM_NUMERICAL_MASK >> i) & 1u)) rows[m * np1 + i] = (CDM_F64)(cdm_lm_acc)(d[i] * dtf[i]);
     rows[m * np1 + cdm_np] = (CDM_F64)(cdm_lm_acc)(sig - (cdm_jt)mv[m]);
   }
+  // The probes' voxel caches, all at once: first the jobs of the probes
+  // whose parameter moves them, pair (probe, job) on its own lane, then each
+  // probe's cache on its own lane, both into LDS.  Every lane building every
+  // probe's cache in turn cost more than the measurement loops.
+  constexpr int nDep = __builtin_popcount(cdm_lm_kdep), nJdep = __builtin_popcount(cdm_lm_kjdep);
+  static_assert(2 * nDep <= L, "one probe cache per lane");
+  CDM_F64* probe_jobs = base_mvc + (size_t)g.nmeas * cdm_lm_base_mvc;  // [2 nJdep][jobs]
+  CDM_F64* probe_vc = probe_jobs + (size_t)2 * nJdep * cdm_vc_jobs;    // [2 nDep][cache]
+  // parameters of the probe on side `side` (0: p_i + h/2, 1: that - 2 (h/2))
+  auto probe_point = [&](int i, int side, cdm_jt* q) {
+    cdm_jt pi = (cdm_jt)0;
+#pragma unroll
+    for (int k = 0; k < cdm_np; ++k)
+      if (k == i) pi = pj[k];
+    cdm_jt h = fabs_gpu((cdm_jt)::cdm_fd_step<cdm_jt>::rel * pi);
+    h = min_gpu(max_gpu(h, (cdm_jt)::cdm_fd_step<cdm_jt>::lo), (cdm_jt)::cdm_fd_step<cdm_jt>::hi);
+    const cdm_jt up = pi + h * (cdm_jt)0.5;
+    const cdm_jt pp = side == 0 ? up : up - (cdm_jt)2 * (h * (cdm_jt)0.5);
+#pragma unroll
+    for (int k = 0; k < cdm_np; ++k) q[k] = k == i ? pp : pj[k];
+  };
+  // index of the k-th parameter in a mask, and the parameters of a mask below i
+  auto nth_param = [](unsigned mask, int k) {
+    int idx = 0, c = 0;
+#pragma unroll
+    for (int b = 0; b < cdm_np; ++b)
+      if ((mask >> b) & 1u) {
+        if (c == k) idx = b;
+        ++c;
+      }
+    return idx;
+  };
+  auto rank_of = [](unsigned mask, int i) {
+    int c = 0;
+#pragma unroll
+    for (int b = 0; b < cdm_np; ++b)
+      if (b < i && ((mask >> b) & 1u)) ++c;
+    return c;
+  };
+  {
+    constexpr int TJ = 2 * nJdep * cdm_vc_jobs;
+#pragma unroll
+    for (int r = 0; r < (TJ + L - 1) / L; ++r) {
+      const int t = lane + r * L;
+      if (t < TJ) {
+        const int pr = t / cdm_vc_jobs;
+        cdm_jt q[cdm_np];
+        probe_point(nth_param(cdm_lm_kjdep, pr / 2), pr % 2, q);
+        probe_jobs[t] = Voxel_Cache_Job(t % cdm_vc_jobs, cdm_np, q, fxj);
+      }
+    }
+  }
+  cdm_wave_lds_sync();
+  if (lane < 2 * nDep) {
+    const int i = nth_param(cdm_lm_kdep, lane / 2), side = lane % 2;
+    cdm_jt q[cdm_np];
+    probe_point(i, side, q);
+    CDM_F64 jq[cdm_vc_jobs], vc[cdm_vc];
+    const bool jdep = (cdm_lm_kjdep >> i) & 1u;
+    const int pr = 2 * rank_of(cdm_lm_kjdep, i) + side;
+#pragma unroll
+    for (int j = 0; j < cdm_vc_jobs; ++j) jq[j] = jdep ? probe_jobs[pr * cdm_vc_jobs + j] : jr0[j];
+    Voxel_Cache_Combine(cdm_np, q, fxj, jq, vc);
+#pragma unroll
+    for (int c = 0; c < cdm_vc; ++c) probe_vc[lane * cdm_vc + c] = vc[c];
+  }
+  cdm_wave_lds_sync();
   // one column per central-difference parameter (a loop, not unrolled: the
   // parameter index selects by compares, the body is one measurement loop)
 #pragma unroll 1
@@ -875,43 +959,18 @@ __device__ __forceinline__ void cdm_lm_jacobian_columns(const cdm_geom& g, const
       }
     cdm_jt h = fabs_gpu((cdm_jt)::cdm_fd_step<cdm_jt>::rel * pi);
     h = min_gpu(max_gpu(h, (cdm_jt)::cdm_fd_step<cdm_jt>::lo), (cdm_jt)::cdm_fd_step<cdm_jt>::hi);
-    const cdm_jt up = pi + h * (cdm_jt)0.5;
-    const cdm_jt dn = up - (cdm_jt)2 * (h * (cdm_jt)0.5);
-    const bool dep = (cdm_vc_dep >> i) & 1u, jdep = (cdm_vcj_dep >> i) & 1u;
-    // the two probes' jobs, pair t = probe * jobs + job, one per lane
-    CDM_F64 mine = 0.0;
-    if (dep && jdep) {
-      constexpr int T = 2 * cdm_vc_jobs;
-      static_assert(T <= L, "one round of probe jobs");
-      const int t = lane < T ? lane : 0;
-      const cdm_jt pt = t < cdm_vc_jobs ? up : dn;
-      cdm_jt q[cdm_np];
-#pragma unroll
-      for (int k = 0; k < cdm_np; ++k) q[k] = k == i ? pt : pj[k];
-      mine = Voxel_Cache_Job(t % cdm_vc_jobs, cdm_np, q, fxj);
-    }
-    // the probes one after the other, so that one probe's cache is live at a
-    // time: the upper probe's signals go to the column's LDS entries (as
-    // doubles, exact for the signal's type), the lower probe's pass forms
-    // the quotient from them
+    const bool dep = (cdm_lm_kdep >> i) & 1u;
+    const int t0 = 2 * rank_of(cdm_lm_kdep, i);
+    // the probes one after the other: the upper probe's signals go to the
+    // column's LDS entries (as doubles, exact for the signal's type), the
+    // lower probe's pass forms the quotient from them
 #pragma unroll 1
     for (int probe = 0; probe < 2; ++probe) {
-      const cdm_jt pp = probe == 0 ? up : dn;
       cdm_jt q[cdm_np];
-#pragma unroll
-      for (int k = 0; k < cdm_np; ++k) q[k] = k == i ? pp : pj[k];
+      probe_point(i, probe, q);
       CDM_F64 vc[cdm_vc];
-      if (!dep) {
 #pragma unroll
-        for (int c = 0; c < cdm_vc; ++c) vc[c] = vc0[c];
-      } else if (jdep) {
-        CDM_F64 jq[cdm_vc_jobs];
-#pragma unroll
-        for (int j = 0; j < cdm_vc_jobs; ++j) jq[j] = cdm::Group<L>::bcast(mine, probe * cdm_vc_jobs + j);
-        Voxel_Cache_Combine(cdm_np, q, fxj, jq, vc);
-      } else {
-        Voxel_Cache_Combine(cdm_np, q, fxj, jr0, vc);
-      }
+      for (int c = 0; c < cdm_vc; ++c) vc[c] = dep ? probe_vc[(t0 + probe) * cdm_vc + c] : vc0[c];
       for (int m = lane; m < g.nmeas; m += L) {
         cdm_jt* cm = rowj.at(s_cfp, m, g.cfp_ts);
 #if defined(CDM_MEAS_VOXEL_CACHE_PARTIAL)
