@@ -61,6 +61,7 @@ def lib():
             "cdm_mcmc": (i, [vp, i, i, vp, vp, vp, vp, i, i, i, i, i, u64, i64, vp, vp, vp]),
             "cdm_mcmc_noalloc": (i, [vp, i, i, vp, vp, vp, vp, i, i, i, i, i, u64, i64, vp, vp, vp]),
             "cdm_model_set_debug": (i, [vp, i, i]),
+            "cdm_batch_set_fill": (i, [i]),
             "cdm_model_debug_trace": (i, [vp, i, vp, i, ip, vp]),
             "cdm_predicted_signal": (i, [vp, i, i, i, vp, vp, vp, vp, vp]),
             "cdm_bic_aic": (i, [vp, i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp]),
@@ -272,6 +273,13 @@ class BatchItem(ctypes.Structure):
                 ("meas", ctypes.c_void_p), ("params", ctypes.c_void_p), ("cfp", ctypes.c_void_p),
                 ("fixp", ctypes.c_void_p), ("iters", ctypes.c_void_p), ("cost", ctypes.c_void_p),
                 ("samples", ctypes.c_void_p), ("tau_samples", ctypes.c_void_p), ("voxel_offset", ctypes.c_longlong)]
+
+
+def batch_set_fill(blocks_per_cu):
+    """Items filling at least blocks_per_cu blocks per CU run on their own
+    kernels in the batched calls (default 8; 0: every item in the persistent
+    launch)."""
+    check(lib().cdm_batch_set_fill(int(blocks_per_cu)))
 
 
 def batch_graph_stats():

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -953,12 +954,17 @@ int check_batch(int nitems, const cdm_batch_item* items, bool mcmc, const cdm::B
 }
 
 // An item whose voxels alone fill the GPU several times over (blocks of its
-// tuned voxel group >= 8 per CU) runs through the per-model kernels at its
+// tuned voxel group >= 8 per CU by default, cdm_batch_set_fill) runs through
+// the per-model kernels at its
 // own tuned launch shape, voxel group and register budget, with the same
 // draws and results; the persistent kernel, whose register allocation is the
 // largest of its models' and sets one occupancy for all of them, takes the
 // smaller items, which it packs together to fill the GPU.
+std::atomic<int> g_fill_blocks_per_cu{8};
+
 bool fills_gpu(const cdm_model* m, int method, int nmeas, int nvox) {
+  const int per_cu = g_fill_blocks_per_cu.load();
+  if (per_cu <= 0) return false;
   int dev = 0, ncu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) {
@@ -966,7 +972,7 @@ bool fills_gpu(const cdm_model* m, int method, int nmeas, int nvox) {
     return false;
   }
   const long long per_block = 256 / group_for(m, method, nmeas, nvox);
-  return ((long long)nvox + per_block - 1) / per_block >= 8LL * ncu;
+  return ((long long)nvox + per_block - 1) / per_block >= (long long)per_cu * ncu;
 }
 
 // Items ordered by estimated work per voxel (heaviest first), so the queue
@@ -1283,6 +1289,12 @@ int cdm_batch_mcmc(int nitems, const cdm_batch_item* items, int nburn, int njump
   if (rest.empty()) return 0;
   return batch_mcmc_persistent((int)rest.size(), rest.data(), nburn, njumps, sampleevery, update_proposal, rician,
                                seed, jumps_per_launch, use_graph, stream);
+}
+
+int cdm_batch_set_fill(int blocks_per_cu) {
+  if (blocks_per_cu < 0) return fail("blocks per CU must be >= 0");
+  g_fill_blocks_per_cu.store(blocks_per_cu);
+  return 0;
 }
 
 int cdm_batch_graph_stats(long long* captures, long long* replays) {

@@ -315,6 +315,12 @@ int cdm_batch_mcmc(int nitems, const cdm_batch_item* items, int nburn, int njump
                    int update_proposal, int rician, unsigned long long seed, int jumps_per_launch, int use_graph,
                    void* stream);
 
+/* Items whose voxels fill at least blocks_per_cu blocks of their voxel
+ * group per CU run through their model's own kernels (default 8); 0 sends
+ * every item through the persistent launch.  Process-wide; no counterpart in
+ * the reference. */
+int cdm_batch_set_fill(int blocks_per_cu);
+
 /* Graphs recorded and replayed by cdm_batch_mcmc in this process (no
  * counterpart in the reference). */
 int cdm_batch_graph_stats(long long* captures, long long* replays);

@@ -132,13 +132,15 @@ def test_batch_mcmc_equals_per_model_mcmc(blocks, graph):
 
 
 @pytest.mark.gpu
-def test_batch_item_that_fills_the_gpu_runs_its_own_kernels():
+@pytest.mark.parametrize("fill", [8, 0])
+def test_batch_item_that_fills_the_gpu_runs_its_own_kernels(fill):
     """A DiffusionTensor item of 140,000 voxels (2,188 blocks of its 4-lane
     voxel groups, over 8 per CU) goes through the per-model kernels at its
     tuned launch shape, the two small items through the persistent launch;
     LM and MCMC results equal the per-model calls' either way, and the MCMC
     jump-block graphs land in the big item's handle and in the batch
-    workspace respectively."""
+    workspace respectively.  With cdm_batch_set_fill(0) every item goes
+    through the persistent launch, with the same results."""
     import torch
     from cudimot_amd import capi, synth
     vols = _volumes(700)
@@ -180,15 +182,22 @@ def test_batch_item_that_fills_the_gpu_runs_its_own_kernels():
                           voxel_offset=off))
         off += v["n"]
     before = capi.batch_graph_stats()
-    capi.batch_levmar(items, max_iters=50)
-    capi.batch_mcmc(items, nburn=40, njumps=20, sampleevery=10, seed=5, jumps_per_launch=20, use_graph=True)
-    capi.sync()
+    capi.batch_set_fill(fill)
+    try:
+        capi.batch_levmar(items, max_iters=50)
+        capi.batch_mcmc(items, nburn=40, njumps=20, sampleevery=10, seed=5, jumps_per_launch=20, use_graph=True)
+        capi.sync()
+    finally:
+        capi.batch_set_fill(8)
     for (p, cost, s), b in zip(ref, items):
         np.testing.assert_array_equal(host(cost), host(b["cost"]))
         np.testing.assert_array_equal(host(s), host(b["samples"]))
         np.testing.assert_array_equal(host(p), host(b["params"]))
-    assert bat_models[0].graph_stats()[0] == 1  # the burn-in's two blocks, recorded on the handle
     assert bat_models[1].graph_stats() == (0, 0) and bat_models[2].graph_stats() == (0, 0)
+    if fill:
+        assert bat_models[0].graph_stats()[0] == 1  # the burn-in's two blocks, recorded on the handle
+    else:
+        assert bat_models[0].graph_stats() == (0, 0)
     assert sum(capi.batch_graph_stats()) == sum(before) + 1
 
 
