@@ -80,58 +80,87 @@ struct Epi {
 // MFMA 32x32x16 operand layout (wave64): lane l supplies A row (l & 31) and
 // B row (l & 31), elements k0 + 8 (l >> 5) .. +7; accumulator element j of
 // lane l is C[8 (j >> 2) + 4 (l >> 5) + (j & 3)][l & 31].
-template <int EPI>
-__global__ void __launch_bounds__(256) gemm_nt(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B,
-                                               int ldb, int M, int N, int K, Epi e) {
+// STEPS > 0: K == 16 STEPS, and every operand of the K range is loaded before
+// the first product, with the epilogue's own inputs (bias, pre-activations),
+// so one memory latency precedes the MFMA chain instead of one per chunk;
+// STEPS == 0: any K, in prefetched chunks.  The products run in the same K
+// order either way.
+// (bx, by, bz): the block's tile column, tile row and K slice.
+template <int EPI, int STEPS>
+__device__ __forceinline__ void gemm_tile(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B, int ldb,
+                                          int M, int N, int K, const Epi& e, int bx, int by, int bz) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int m0 = blockIdx.y * kTile + (wave >> 1) * 32;
-  const int n0 = blockIdx.x * kTile + (wave & 1) * 32;
+  const int m0 = by * kTile + (wave >> 1) * 32;
+  const int n0 = bx * kTile + (wave & 1) * 32;
   if (m0 >= M || n0 >= N) return;
   const int r = lane & 31, half = lane >> 5;
-  const size_t kb = (size_t)blockIdx.z * K;
+  const size_t kb = (size_t)bz * K;
   const bf16* pa = A + (size_t)(m0 + r) * lda + kb + 8 * half;
   const bf16* pb = B + (size_t)(n0 + r) * ldb + kb + 8 * half;
+  const int col = n0 + r;
+  const float bias = (EPI != EPI_BWD_GELU && e.bias) ? e.bias[col] : 0.0f;
   f32x16 acc = {};
-  // K in chunks of four MFMA steps, the next chunk's operands loaded before
-  // the current chunk's products (each step's load would otherwise wait for
-  // the memory latency alone), then the steps that remain
-  constexpr int kChunk = 4 * 16;
-  int k0 = 0;
-  if (K >= kChunk) {
-    bf16x8 a[4], b[4];
+  float zpre[16];
+  if constexpr (STEPS > 0) {
+    bf16x8 a[STEPS], b[STEPS];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < STEPS; ++u) {
       a[u] = *(const bf16x8*)(pa + 16 * u);
       b[u] = *(const bf16x8*)(pb + 16 * u);
     }
-    for (; k0 + kChunk <= K; k0 += kChunk) {
-      bf16x8 an[4], bn[4];
-      const bool more = k0 + 2 * kChunk <= K;
-      if (more) {
+    if (EPI == EPI_BWD_GELU) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          an[u] = *(const bf16x8*)(pa + k0 + kChunk + 16 * u);
-          bn[u] = *(const bf16x8*)(pb + k0 + kChunk + 16 * u);
-        }
+      for (int j = 0; j < 16; ++j) zpre[j] = e.zin[(size_t)(m0 + 8 * (j >> 2) + 4 * half + (j & 3)) * e.ld_zin + col];
+    }
+    // keep every load above ahead of the products (the scheduler would
+    // otherwise interleave them to save registers)
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < STEPS; ++u) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[u], b[u], acc, 0, 0, 0);
+  } else {
+    // K in chunks of four MFMA steps, the next chunk's operands loaded before
+    // the current chunk's products (each step's load would otherwise wait for
+    // the memory latency alone), then the steps that remain
+    constexpr int kChunk = 4 * 16;
+    int k0 = 0;
+    if (K >= kChunk) {
+      bf16x8 a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = *(const bf16x8*)(pa + 16 * u);
+        b[u] = *(const bf16x8*)(pb + 16 * u);
       }
+      for (; k0 + kChunk <= K; k0 += kChunk) {
+        bf16x8 an[4], bn[4];
+        const bool more = k0 + 2 * kChunk <= K;
+        if (more) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[u], b[u], acc, 0, 0, 0);
-      if (more) {
+          for (int u = 0; u < 4; ++u) {
+            an[u] = *(const bf16x8*)(pa + k0 + kChunk + 16 * u);
+            bn[u] = *(const bf16x8*)(pb + k0 + kChunk + 16 * u);
+          }
+        }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          a[u] = an[u];
-          b[u] = bn[u];
+        for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[u], b[u], acc, 0, 0, 0);
+        if (more) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            a[u] = an[u];
+            b[u] = bn[u];
+          }
         }
       }
     }
+    for (; k0 < K; k0 += 16) {
+      const bf16x8 a = *(const bf16x8*)(pa + k0);
+      const bf16x8 b = *(const bf16x8*)(pb + k0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    if (EPI == EPI_BWD_GELU) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) zpre[j] = e.zin[(size_t)(m0 + 8 * (j >> 2) + 4 * half + (j & 3)) * e.ld_zin + col];
+    }
   }
-  for (; k0 < K; k0 += 16) {
-    const bf16x8 a = *(const bf16x8*)(pa + k0);
-    const bf16x8 b = *(const bf16x8*)(pb + k0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-  }
-  const int col = n0 + r;
-  float bias = (EPI != EPI_BWD_GELU && e.bias) ? e.bias[col] : 0.0f;
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
     const int row0 = m0 + 8 * g + 4 * half;
@@ -141,13 +170,13 @@ __global__ void __launch_bounds__(256) gemm_nt(const bf16* __restrict__ A, int l
       const int row = row0 + i;
       float v = acc[4 * g + i];
       if (EPI == EPI_PLAIN) {
-        e.out_f32[blockIdx.z * e.split_stride + (size_t)row * e.ld_f32 + col] = v + bias;
+        e.out_f32[bz * e.split_stride + (size_t)row * e.ld_f32 + col] = v + bias;
       } else if (EPI == EPI_FWD_GELU) {
         const float z = v + bias;
         if (e.out_f32) e.out_f32[(size_t)row * e.ld_f32 + col] = z;
         v = gelu(z);
       } else {
-        v *= gelu_grad(e.zin[(size_t)row * e.ld_zin + col]);
+        v *= gelu_grad(zpre[4 * g + i]);
       }
       if (EPI != EPI_PLAIN) {
         const bf16 h = (bf16)v;
@@ -157,6 +186,43 @@ __global__ void __launch_bounds__(256) gemm_nt(const bf16* __restrict__ A, int l
     }
     if (EPI != EPI_PLAIN && e.out_bft) *(bf16x4*)(e.out_bft + (size_t)col * e.ld_bft + row0) = tv;
   }
+}
+
+template <int EPI, int STEPS>
+__global__ void __launch_bounds__(256) gemm_nt(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B,
+                                               int ldb, int M, int N, int K, Epi e) {
+  gemm_tile<EPI, STEPS>(A, lda, B, ldb, M, N, K, e, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Several independent plain products with the same K slice length in one
+// launch (the weight gradients of every layer): problem p owns the blocks
+// [first[p], first[p] + tiles_n tiles_m splits), tile column fastest, then
+// tile row, then K slice.
+struct GemmJob {
+  const bf16* A;
+  const bf16* B;
+  int lda, ldb, M, N, tiles_n, tiles_m, first;
+  Epi e;
+};
+// advance (a training-step graph's step index, read by no block of this
+// launch): its first thread moves it on to the next step's entry.
+struct GemmJobs {
+  int n, K;
+  int* advance;
+  GemmJob j[kMaxLayers];
+};
+
+template <int STEPS>
+__global__ void __launch_bounds__(256) gemm_multi(GemmJobs jobs) {
+  if (jobs.advance && blockIdx.x == 0 && threadIdx.x == 0) *jobs.advance = *jobs.advance + 1;
+  int b = blockIdx.x, p = 0;
+  while (p + 1 < jobs.n && b >= jobs.j[p + 1].first) ++p;
+  const GemmJob& J = jobs.j[p];
+  b -= J.first;
+  const int bx = b % J.tiles_n;
+  b /= J.tiles_n;
+  const int by = b % J.tiles_m, bz = b / J.tiles_m;
+  gemm_tile<EPI_PLAIN, STEPS>(J.A, J.lda, J.B, J.ldb, J.M, J.N, jobs.K, J.e, bx, by, bz);
 }
 
 // ------------------------------------------------------------------ inputs
@@ -180,7 +246,8 @@ __global__ void inputs_kernel(int nrows, int rows_pad, int nmeas, int kpad, cons
 // Arguments of one training step that change from step to step: the batch's
 // first sample, the epoch (noise stream), the learning rate and Adam's bias
 // corrections.  A replayed graph reads them from a device table at the
-// index *ctr, which its last kernel advances; eager launches pass tab null.
+// index *ctr, which its weight-gradient launch advances (the Adam step after
+// it reads the entry behind); eager launches pass tab null.
 struct StepArgs {
   long long first;
   unsigned ep;
@@ -258,18 +325,26 @@ __global__ void encode_kernel(int n, int nparams, const float* __restrict__ para
 
 // Loss and its gradient with respect to the raw outputs (times grad_scale,
 // the inverse batch size), written as BF16 in both layouts (padding columns
-// zero); loss_sum accumulates the loss times loss_scale.
-__global__ void loss_kernel(int rows, int n_out, int npad, const float* __restrict__ out,
-                            const float* __restrict__ target, Head h, float grad_scale, float loss_scale, bf16* dout,
-                            bf16* doutt, float* loss_sum, const StepArgs* tab, const int* ctr) {
+// zero); loss_sum accumulates the loss times loss_scale.  One thread per
+// sample, one wave per block (the head's outputs [0, n_out) are covered by
+// its entries in order; each gradient value is stored as it is formed).
+constexpr int kLossBlock = 64;
+__global__ void __launch_bounds__(kLossBlock) loss_kernel(int rows, int n_out, int npad, const float* __restrict__ out,
+                                                          const float* __restrict__ target, Head h, float grad_scale,
+                                                          float loss_scale, bf16* dout, bf16* doutt, float* loss_sum,
+                                                          const StepArgs* tab, const int* ctr) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (tab) target += (size_t)tab[*ctr].first * n_out;
   float l = 0.0f;
   if (v < rows) {
     const float* o = out + (size_t)v * npad;
     const float* t = target + (size_t)v * n_out;
-    float g[32];
-    for (int k = 0; k < npad; ++k) g[k] = 0.0f;
+    bf16* d = dout + (size_t)v * npad;
+    auto put = [&](int k, float g) {
+      const bf16 q = (bf16)(g * grad_scale);
+      d[k] = q;
+      doutt[(size_t)k * rows + v] = q;
+    };
     for (int j = 0; j < h.n; ++j) {
       const cdm_head_entry& e = h.e[j];
       const int a = h.out_index[j];
@@ -279,33 +354,23 @@ __global__ void loss_kernel(int rows, int n_out, int npad, const float* __restri
         const float rr = vx * vx + vy * vy + vz * vz + 1e-6f;
         l += e.weight * (1.0f - q * q / rr);
         const float c1 = -2.0f * e.weight * q / rr, c2 = q / rr;
-        g[a] = c1 * (t[a] - c2 * vx);
-        g[a + 1] = c1 * (t[a + 1] - c2 * vy);
-        g[a + 2] = c1 * (t[a + 2] - c2 * vz);
+        put(a, c1 * (t[a] - c2 * vx));
+        put(a + 1, c1 * (t[a + 1] - c2 * vy));
+        put(a + 2, c1 * (t[a + 2] - c2 * vz));
       } else {
         const float s = sigmoid(o[a]), tt = t[a];
         const float sc = fminf(fmaxf(s, 1e-7f), 1.0f - 1e-7f);
         // soft-target cross entropy minus its minimum (0 at s = t)
         const float tc = fminf(fmaxf(tt, 1e-7f), 1.0f - 1e-7f);
         l += e.weight * (tc * logf(tc / sc) + (1.0f - tc) * logf((1.0f - tc) / (1.0f - sc)));
-        g[a] = e.weight * (s - tt);
+        put(a, e.weight * (s - tt));
       }
     }
-    for (int k = 0; k < npad; ++k) {
-      const bf16 d = (bf16)(g[k] * grad_scale);
-      dout[(size_t)v * npad + k] = d;
-      doutt[(size_t)k * rows + v] = d;
-    }
+    for (int k = n_out; k < npad; ++k) put(k, 0.0f);
   }
-  // block sum of the loss, one atomic per block (global memory, vector path)
-  __shared__ float red[256];
-  red[threadIdx.x] = l;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicAdd(loss_sum, red[0] * loss_scale);
+  // wave sum of the loss, one atomic per block (global memory, vector path)
+  for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
+  if (threadIdx.x == 0) atomicAdd(loss_sum, l * loss_scale);
 }
 
 // Raw outputs -> model parameters, entries in order (a ratio entry reads the
@@ -338,92 +403,116 @@ __global__ void decode_kernel(int n, int npad, const float* __restrict__ out, He
   }
 }
 
-// One launch per layer of the weight gradient's second stage and the bias
-// gradient: blocks [0, ceil(n/256)) sum the S partial products in order,
-// gw[i] = sum_z part[z][i]; block nb + u forms db[u] = sum_b dzt[u][b]
-// (rows a multiple of 8: each thread sums eight-element vectors, then the
-// block reduces in a fixed order).
-__global__ void __launch_bounds__(256) wgrad_finish_kernel(int n, int S, const float* __restrict__ part, float* gw,
-                                                           int nunits, int rows, const bf16* __restrict__ dzt,
-                                                           float* db) {
-  const int nb = (n + 255) / 256;
-  if ((int)blockIdx.x < nb) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    float acc = part[i];
-    for (int z = 1; z < S; ++z) acc += part[(size_t)z * n + i];
-    gw[i] = acc;
-    return;
-  }
-  const int unit = blockIdx.x - nb;
-  if (unit >= nunits) return;
-  const bf16* row = dzt + (size_t)unit * rows;
-  float s = 0.0f;
-  for (int b = 8 * threadIdx.x; b < rows; b += 8 * 256) {
-    const bf16x8 q = *(const bf16x8*)(row + b);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) s += (float)q[j];
-  }
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-  __shared__ float red[4];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) db[unit] = (red[0] + red[1]) + (red[2] + red[3]);
-}
-
-// Adam over every weight matrix [nrow][ncol] (padded storage, live region
-// [rows_live][cols_live]) and bias vector (ncol = 1) of the network in one
-// launch: FP32 master, moments, and the BF16 copies of the matrices in both
-// layouts.  Tensor t owns the flat indices [first[t], first[t + 1]).
+// Adam over weight matrices [nrow][ncol] (padded storage, live region
+// [rows_live][cols_live]) and bias vectors (ncol = 1): FP32 master, moments,
+// and the BF16 copies of the matrices in both layouts.
 struct AdamTensor {
   int nrow, ncol, rows_live, cols_live;
   float *w, *m, *v;
   const float* g;
   bf16 *wbf, *wbft;
 };
-struct AdamSet {
-  int nt;
-  int first[2 * kMaxLayers + 1];
-  AdamTensor t[2 * kMaxLayers];
+struct AdamArgs {
+  float lr, b1, b2, eps, c1, c2;
 };
 
-__global__ void adam_kernel(AdamSet set, float lr, float b1, float b2, float eps, float c1, float c2,
-                            const StepArgs* tab, const int* ctr) {
-  const int gi_ = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gi_ >= set.first[set.nt]) return;
-  if (tab) {
-    lr = tab[*ctr].lr;
-    c1 = tab[*ctr].c1;
-    c2 = tab[*ctr].c2;
-  }
-  int k = 0;
-  while (k + 1 < set.nt && gi_ >= set.first[k + 1]) ++k;
-  const AdamTensor& T = set.t[k];
-  const int i = gi_ - set.first[k];
-  const int nrow = T.nrow, ncol = T.ncol;
-  float *w = T.w, *m = T.m, *v = T.v;
-  const float* g = T.g;
-  bf16 *wbf = T.wbf, *wbft = T.wbft;
-  const int r = i / ncol, c = i - r * ncol;
-  if (r >= T.rows_live || c >= T.cols_live) return;  // padding stays zero
-  const float gi = g[i];
-  const float mi = b1 * m[i] + (1.0f - b1) * gi;
-  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
-  m[i] = mi;
-  v[i] = vi;
-  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
-  w[i] = wi;
-  if (wbf) {
+// Element i's weight and moments, loaded ahead of its gradient.
+struct AdamElem {
+  float w, m, v;
+};
+__device__ __forceinline__ AdamElem adam_load(const AdamTensor& T, int i) { return {T.w[i], T.m[i], T.v[i]}; }
+
+// Element i of tensor T (weight and moments x) with gradient gi; padding
+// stays zero.  The moment updates name their fused multiply-adds, so every
+// kernel that inlines this rounds them alike.
+__device__ __forceinline__ void adam_update(const AdamTensor& T, int i, float gi, AdamElem x, const AdamArgs& a) {
+  const int r = i / T.ncol, c = i - r * T.ncol;
+  if (r >= T.rows_live || c >= T.cols_live) return;
+  const float mi = fmaf(a.b1, x.m, (1.0f - a.b1) * gi);
+  const float vi = fmaf(a.b2, x.v, (1.0f - a.b2) * gi * gi);
+  T.m[i] = mi;
+  T.v[i] = vi;
+  const float wi = x.w - a.lr * (mi / a.c1) / (sqrtf(vi / a.c2) + a.eps);
+  T.w[i] = wi;
+  if (T.wbf) {
     const bf16 h = (bf16)wi;
-    wbf[i] = h;
-    wbft[(size_t)c * nrow + r] = h;
+    T.wbf[i] = h;
+    T.wbft[(size_t)c * T.nrow + r] = h;
   }
 }
 
-// The last node of a training-step graph: the next replay reads the next
-// step's arguments.
-__global__ void step_advance_kernel(int* ctr) {
-  if (threadIdx.x == 0) ctr[0] = ctr[0] + 1;
+// A replayed graph takes lr and the bias corrections from the step table at
+// the entry behind the step index (advanced by the weight-gradient launch
+// before it).
+__device__ __forceinline__ AdamArgs adam_args(AdamArgs a, const StepArgs* tab, const int* ctr) {
+  if (tab) {
+    const StepArgs& t = tab[*ctr - 1];
+    a.lr = t.lr;
+    a.c1 = t.c1;
+    a.c2 = t.c2;
+  }
+  return a;
+}
+
+// The weight gradients' second stage and the bias gradients of every layer in
+// one launch.  Layer p owns the blocks [first, first + nb + units): blocks
+// [0, nb) of it sum the S partial products in order, gw[i] = sum_z
+// part[z][i] (nb = ceil(n / 256)); block nb + u forms db[u] = sum_b
+// dzt[u][b] (rows a multiple of 8: each thread sums eight-element vectors,
+// then the block reduces in a fixed order).  With tw.w set, each gradient
+// element also takes its Adam step there (tw: the layer's weights, tb: its
+// bias), with lr and the corrections from tab[*ctr - 1] when tab is set.
+struct FinishLayer {
+  int n, units, nb, first;
+  const float* part;
+  float* gw;
+  const bf16* dzt;
+  float* db;
+  AdamTensor tw, tb;
+};
+struct FinishSet {
+  int nl, S, rows;
+  FinishLayer L[kMaxLayers];
+};
+
+__global__ void __launch_bounds__(256) wgrad_finish_kernel(FinishSet fs, AdamArgs aa, const StepArgs* tab,
+                                                           const int* ctr) {
+  int p = 0;
+  while (p + 1 < fs.nl && (int)blockIdx.x >= fs.L[p + 1].first) ++p;
+  const FinishLayer& F = fs.L[p];
+  const int blk = blockIdx.x - F.first;
+  const AdamArgs a = F.tw.w ? adam_args(aa, tab, ctr) : aa;
+  if (blk < F.nb) {
+    const int i = blk * 256 + threadIdx.x;
+    if (i < F.n) {
+      AdamElem x = {0.0f, 0.0f, 0.0f};
+      if (F.tw.w) x = adam_load(F.tw, i);
+      float acc = F.part[i];
+      for (int z = 1; z < fs.S; ++z) acc += F.part[(size_t)z * F.n + i];
+      F.gw[i] = acc;
+      if (F.tw.w) adam_update(F.tw, i, acc, x, a);
+    }
+  } else {
+    const int unit = blk - F.nb;
+    float s = 0.0f;
+    if (unit < F.units) {
+      const bf16* row = F.dzt + (size_t)unit * fs.rows;
+      for (int b = 8 * threadIdx.x; b < fs.rows; b += 8 * 256) {
+        const bf16x8 q = *(const bf16x8*)(row + b);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s += (float)q[j];
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    __shared__ float red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0 && unit < F.units) {
+      const float g = (red[0] + red[1]) + (red[2] + red[3]);
+      F.db[unit] = g;
+      if (F.tw.w) adam_update(F.tb, unit, g, adam_load(F.tb, unit), a);
+    }
+  }
 }
 
 __global__ void to_bf16_kernel(int nrow, int ncol, const float* w, bf16* wbf, bf16* wbft) {
@@ -458,11 +547,12 @@ struct cdm_mlp {
   float* z[kMaxLayers] = {};
   bf16 *h[kMaxLayers] = {}, *ht[kMaxLayers] = {};
   float* out = nullptr;
-  bf16 *dout = nullptr, *doutt = nullptr, *dz[2] = {}, *dzt[2] = {};
+  bf16 *dout = nullptr, *doutt = nullptr, *dz[2] = {};
+  bf16* dzt[kMaxLayers] = {};  // per hidden layer: dZ^T, kept for the weight gradients
   float* loss_sum = nullptr;
   bf16 *ix = nullptr, *ih[2] = {};
   float* iout = nullptr;
-  float* wpart = nullptr;  // split-K partial weight gradients (kWgradSplits x the largest layer)
+  float* wpart = nullptr;  // split-K partial weight gradients (kWgradSplits x every layer)
   // training steps replayed from one captured graph (the step table and its
   // index), or launched one by one
   bool train_graph = true;
@@ -484,11 +574,11 @@ void free_all(cdm_mlp* m) {
   for (int l = 0; l < kMaxLayers; ++l) {
     for (float* p : {m->w[l], m->b[l], m->gw[l], m->gb[l], m->mw[l], m->vw[l], m->mb[l], m->vb[l], m->z[l]})
       if (p) (void)hipFree(p);
-    for (bf16* p : {m->wbf[l], m->wbft[l], m->h[l], m->ht[l]})
+    for (bf16* p : {m->wbf[l], m->wbft[l], m->h[l], m->ht[l], m->dzt[l]})
       if (p) (void)hipFree(p);
   }
   for (void* p : {(void*)m->x, (void*)m->xt, (void*)m->out, (void*)m->dout, (void*)m->doutt, (void*)m->dz[0],
-                  (void*)m->dz[1], (void*)m->dzt[0], (void*)m->dzt[1], (void*)m->loss_sum, (void*)m->ix,
+                  (void*)m->dz[1], (void*)m->loss_sum, (void*)m->ix,
                   (void*)m->ih[0], (void*)m->ih[1], (void*)m->iout, (void*)m->wpart})
     if (p) (void)hipFree(p);
 }
@@ -522,14 +612,15 @@ int ensure_train(cdm_mlp* m, int rows) {
   if (m->train_rows >= rows) return 0;
   hipError_t e = hipSuccess;
   for (void* p : {(void*)m->x, (void*)m->xt, (void*)m->out, (void*)m->dout, (void*)m->doutt, (void*)m->dz[0],
-                  (void*)m->dz[1], (void*)m->dzt[0], (void*)m->dzt[1]})
+                  (void*)m->dz[1]})
     if (p) (void)hipFree(p);
   for (int l = 0; l < kMaxLayers; ++l) {
     if (m->z[l]) (void)hipFree(m->z[l]);
     if (m->h[l]) (void)hipFree(m->h[l]);
     if (m->ht[l]) (void)hipFree(m->ht[l]);
+    if (m->dzt[l]) (void)hipFree(m->dzt[l]);
     m->z[l] = nullptr;
-    m->h[l] = m->ht[l] = nullptr;
+    m->h[l] = m->ht[l] = m->dzt[l] = nullptr;
   }
   const size_t W = m->width;
   if (e == hipSuccess) e = dalloc(&m->x, (size_t)rows * m->kpad);
@@ -542,14 +633,12 @@ int ensure_train(cdm_mlp* m, int rows) {
   if (e == hipSuccess) e = dalloc(&m->out, (size_t)rows * m->npad);
   if (e == hipSuccess) e = dalloc(&m->dout, (size_t)rows * m->npad);
   if (e == hipSuccess) e = dalloc(&m->doutt, (size_t)rows * m->npad);
-  for (int i = 0; i < 2 && e == hipSuccess; ++i) {
-    e = dalloc(&m->dz[i], rows * W);
-    if (e == hipSuccess) e = dalloc(&m->dzt[i], rows * W);
-  }
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) e = dalloc(&m->dz[i], rows * W);
+  for (int l = 0; l < m->depth && e == hipSuccess; ++l) e = dalloc(&m->dzt[l], rows * W);
   if (!m->wpart && e == hipSuccess) {
-    size_t most = 0;
-    for (int l = 0; l <= m->depth; ++l) most = std::max(most, (size_t)m->rows[l] * m->cols[l]);
-    e = dalloc(&m->wpart, (size_t)kWgradSplits * most);
+    size_t all = 0;
+    for (int l = 0; l <= m->depth; ++l) all += (size_t)m->rows[l] * m->cols[l];
+    e = dalloc(&m->wpart, (size_t)kWgradSplits * all);
   }
   if (e != hipSuccess) {
     m->train_rows = 0;
@@ -571,11 +660,19 @@ int ensure_infer(cdm_mlp* m) {
   return 0;
 }
 
+// The K range of one block picks the kernel: 2, 4, 8 or 16 MFMA steps are
+// loaded whole, other lengths run in chunks.
 template <int EPI>
 void gemm(const bf16* A, int lda, const bf16* B, int ldb, int M, int N, int K, const Epi& e, hipStream_t s,
           int splits = 1) {
-  hipLaunchKernelGGL(gemm_nt<EPI>, dim3((N + kTile - 1) / kTile, (M + kTile - 1) / kTile, splits), dim3(256), 0, s, A,
-                     lda, B, ldb, M, N, K / splits, e);
+  const dim3 grid((N + kTile - 1) / kTile, (M + kTile - 1) / kTile, splits);
+  const int ks = K / splits;
+  auto kernel = gemm_nt<EPI, 0>;
+  if (ks == 32) kernel = gemm_nt<EPI, 2>;
+  else if (ks == 64) kernel = gemm_nt<EPI, 4>;
+  else if (ks == 128) kernel = gemm_nt<EPI, 8>;
+  else if (ks == 256) kernel = gemm_nt<EPI, 16>;
+  hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, A, lda, B, ldb, M, N, ks, e);
 }
 
 
@@ -595,19 +692,82 @@ int wgrad_splits(int rows) {
   return sp;
 }
 
-// dW = dzt h^T as K slices into m->wpart, then their ordered sum and the bias
-// gradient (sum over the batch of dzt) in one launch.
-void wgrad(cdm_mlp* m, const bf16* dzt, int units, const bf16* hin_t, int cols, int rows, float* gw, float* gb,
-           hipStream_t s) {
-  const int sp = wgrad_splits(rows);
-  Epi g = epi0();
-  g.out_f32 = m->wpart;
-  g.ld_f32 = cols;
-  g.split_stride = (size_t)units * cols;
-  gemm<EPI_PLAIN>(dzt, rows, hin_t, rows, units, cols, rows, g, s, sp);
-  const int n = units * cols;
-  hipLaunchKernelGGL(wgrad_finish_kernel, dim3((n + 255) / 256 + units), dim3(256), 0, s, n, sp, m->wpart, gw, units,
-                     rows, dzt, gb);
+// Adam's step for one training step: hyper-parameters and bias corrections,
+// and, for a replayed graph, the step table, its index and the last-block
+// counter of the kernel that advances the index.
+struct AdamStep {
+  AdamArgs a;
+  const StepArgs* tab;
+  int* ctr;
+};
+
+// dW_l = dZ_l^T H_{l-1} of every layer as K slices into m->wpart, in one
+// launch, then their ordered sums and the bias gradients (sums over the
+// batch of dZ_l^T) in another, which also takes every layer's Adam step when
+// `step` is given (and advances the step index of a replayed graph).
+void wgrad_all(cdm_mlp* m, int rows, const AdamStep* step, hipStream_t s) {
+  const int sp = wgrad_splits(rows), ks = rows / sp;
+  GemmJobs jobs;
+  std::memset(&jobs, 0, sizeof(jobs));
+  FinishSet fs;
+  std::memset(&fs, 0, sizeof(fs));
+  jobs.n = fs.nl = m->depth + 1;
+  jobs.K = ks;
+  fs.S = sp;
+  fs.rows = rows;
+  int gblocks = 0, fblocks = 0;
+  size_t off = 0;
+  for (int l = 0; l <= m->depth; ++l) {
+    const int units = m->rows[l], cols = m->cols[l];
+    const bf16* dzt = l == m->depth ? m->doutt : m->dzt[l];
+    GemmJob& J = jobs.j[l];
+    J.A = dzt;
+    J.lda = rows;
+    J.B = l == 0 ? m->xt : m->ht[l - 1];
+    J.ldb = rows;
+    J.M = units;
+    J.N = cols;
+    J.tiles_n = (cols + kTile - 1) / kTile;
+    J.tiles_m = (units + kTile - 1) / kTile;
+    J.first = gblocks;
+    J.e = epi0();
+    J.e.out_f32 = m->wpart + off;
+    J.e.ld_f32 = cols;
+    J.e.split_stride = (size_t)units * cols;
+    gblocks += J.tiles_n * J.tiles_m * sp;
+    FinishLayer& F = fs.L[l];
+    F.n = units * cols;
+    F.units = units;
+    F.nb = (F.n + 255) / 256;
+    F.first = fblocks;
+    F.part = m->wpart + off;
+    F.gw = m->gw[l];
+    F.dzt = dzt;
+    F.db = m->gb[l];
+    if (step) {
+      F.tw = {m->rows[l], m->cols[l], m->rows_live[l], m->cols_live[l], m->w[l], m->mw[l], m->vw[l], m->gw[l],
+              m->wbf[l], m->wbft[l]};
+      F.tb = {m->rows[l], 1, m->rows_live[l], 1, m->b[l], m->mb[l], m->vb[l], m->gb[l], nullptr, nullptr};
+    }
+    fblocks += F.nb + units;
+    off += (size_t)sp * units * cols;
+  }
+  auto kernel = gemm_multi<0>;
+  if (ks == 64) kernel = gemm_multi<4>;
+  else if (ks == 128) kernel = gemm_multi<8>;
+  else if (ks == 256) kernel = gemm_multi<16>;
+  AdamArgs aa;
+  std::memset(&aa, 0, sizeof(aa));
+  const StepArgs* tab = nullptr;
+  int* ctr = nullptr;
+  if (step) {
+    aa = step->a;
+    tab = step->tab;
+    ctr = step->ctr;
+  }
+  jobs.advance = tab ? ctr : nullptr;
+  hipLaunchKernelGGL(kernel, dim3(gblocks), dim3(256), 0, s, jobs);
+  hipLaunchKernelGGL(wgrad_finish_kernel, dim3(fblocks), dim3(256), 0, s, fs, aa, tab, (const int*)ctr);
 }
 
 // Forward over `rows` (multiple of 32) rows of x.  Training keeps z, h, ht.
@@ -634,15 +794,13 @@ void forward_train(cdm_mlp* m, int rows, hipStream_t s) {
   gemm<EPI_PLAIN>(in, ld_in, m->wbf[m->depth], m->width, rows, m->npad, k, e, s);
 }
 
-// Backward from dout/doutt; fills gw, gb.
-void backward(cdm_mlp* m, int rows, hipStream_t s) {
+// Backward from dout/doutt: dZ_l = (dZ_{l+1} W_{l+1}) gelu'(Z_l) down the
+// layers (the old weights), then every weight and bias gradient (gw, gb),
+// with Adam's step on every layer when `step` is given.
+void backward(cdm_mlp* m, int rows, hipStream_t s, const AdamStep* step = nullptr) {
   const int L = m->depth;
-  // output layer
-  wgrad(m, m->doutt, m->npad, m->ht[L - 1], m->width, rows, m->gw[L], m->gb[L], s);
-  // dH_{L-1} = dout W_L, through gelu' -> dz
   const bf16* dcur = m->dout;
-  int ld_dcur = m->npad, kcur = m->npad;
-  const bf16* wt = m->wbft[L];
+  int kcur = m->npad;
   for (int l = L - 1; l >= 0; --l) {
     const int slot = l & 1;
     Epi e = epi0();
@@ -650,16 +808,13 @@ void backward(cdm_mlp* m, int rows, hipStream_t s) {
     e.ld_zin = m->width;
     e.out_bf = m->dz[slot];
     e.ld_bf = m->width;
-    e.out_bft = m->dzt[slot];
+    e.out_bft = m->dzt[l];
     e.ld_bft = rows;
-    gemm<EPI_BWD_GELU>(dcur, ld_dcur, wt, kcur, rows, m->width, kcur, e, s);
-    // dW_l = dz^T h_{l-1}
-    const bf16* hin_t = l > 0 ? m->ht[l - 1] : m->xt;
-    wgrad(m, m->dzt[slot], m->width, hin_t, m->cols[l], rows, m->gw[l], m->gb[l], s);
+    gemm<EPI_BWD_GELU>(dcur, kcur, m->wbft[l + 1], kcur, rows, m->width, kcur, e, s);
     dcur = m->dz[slot];
-    ld_dcur = kcur = m->width;
-    wt = m->wbft[l];
+    kcur = m->width;
   }
+  wgrad_all(m, rows, step, s);
 }
 
 constexpr float kAdamB1 = 0.9f, kAdamB2 = 0.999f, kAdamEps = 1e-8f;
@@ -668,28 +823,15 @@ constexpr float kAdamB1 = 0.9f, kAdamB2 = 0.999f, kAdamEps = 1e-8f;
 float adam_c1(long long t) { return 1.0f - std::pow(kAdamB1, (float)t); }
 float adam_c2(long long t) { return 1.0f - std::pow(kAdamB2, (float)t); }
 
-// One Adam step over every tensor; with tab (a replayed graph) lr and the
-// corrections come from the step table instead of the arguments.
-void adam(cdm_mlp* m, float lr, hipStream_t s, const StepArgs* tab = nullptr, const int* ctr = nullptr) {
+// The next Adam step's arguments; with tab (a replayed graph) lr and the
+// corrections come from the step table at *ctr instead.
+AdamStep adam_step_args(cdm_mlp* m, float lr, const StepArgs* tab, int* ctr) {
   ++m->adam_step;
-  const float c1 = adam_c1(m->adam_step), c2 = adam_c2(m->adam_step);
-  AdamSet set;
-  std::memset(&set, 0, sizeof(set));
-  int total = 0;
-  for (int l = 0; l <= m->depth; ++l) {
-    AdamTensor& tw = set.t[set.nt];
-    tw = {m->rows[l], m->cols[l], m->rows_live[l], m->cols_live[l], m->w[l], m->mw[l], m->vw[l], m->gw[l],
-          m->wbf[l], m->wbft[l]};
-    set.first[set.nt++] = total;
-    total += m->rows[l] * m->cols[l];
-    AdamTensor& tb = set.t[set.nt];
-    tb = {m->rows[l], 1, m->rows_live[l], 1, m->b[l], m->mb[l], m->vb[l], m->gb[l], nullptr, nullptr};
-    set.first[set.nt++] = total;
-    total += m->rows[l];
-  }
-  set.first[set.nt] = total;
-  hipLaunchKernelGGL(adam_kernel, grid1(total), dim3(256), 0, s, set, lr, kAdamB1, kAdamB2, kAdamEps, c1, c2, tab,
-                     ctr);
+  AdamStep st;
+  st.a = {lr, kAdamB1, kAdamB2, kAdamEps, adam_c1(m->adam_step), adam_c2(m->adam_step)};
+  st.tab = tab;
+  st.ctr = ctr;
+  return st;
 }
 
 // Inference over n voxels in chunks: raw outputs to out [n][npad] (device,
@@ -923,7 +1065,7 @@ int cdm_mlp_loss_grad(cdm_mlp* m, int n, int nmeas, const float* meas, const flo
                      m->x, m->xt);
   forward_train(m, n, s);
   (void)hipMemsetAsync(m->loss_sum, 0, sizeof(float), s);
-  hipLaunchKernelGGL(loss_kernel, grid1(n), dim3(256), 0, s, n, m->n_out, m->npad, m->out, target, m->head,
+  hipLaunchKernelGGL(loss_kernel, grid1(n, kLossBlock), dim3(kLossBlock), 0, s, n, m->n_out, m->npad, m->out, target, m->head,
                      1.0f / n, 1.0f / n, m->dout, m->doutt, m->loss_sum, (const StepArgs*)nullptr, (const int*)nullptr);
   backward(m, n, s);
   e = hipStreamSynchronize(s);
@@ -975,21 +1117,22 @@ int cdm_mlp_train(cdm_mlp* m, int n, int nmeas, const float* clean, const float*
     a.c1 = adam_c1(m->adam_step + st + 1);
     a.c2 = adam_c2(m->adam_step + st + 1);
   }
-  auto step_kernels = [&](hipStream_t q, const StepArgs& a, const StepArgs* tab, const int* ctr) {
+  auto step_kernels = [&](hipStream_t q, const StepArgs& a, const StepArgs* tab, int* ctr) {
     hipLaunchKernelGGL(noisy_inputs_kernel, grid1((long long)batch * m->kpad), dim3(256), 0, q, batch, nmeas,
                        m->kpad, a.first, clean, scale, snr_lo, snr_hi, (uint64_t)seed, (uint32_t)a.ep, m->x, m->xt,
                        tab, ctr);
     forward_train(m, batch, q);
-    hipLaunchKernelGGL(loss_kernel, grid1(batch), dim3(256), 0, q, batch, m->n_out, m->npad, m->out,
+    hipLaunchKernelGGL(loss_kernel, grid1(batch, kLossBlock), dim3(kLossBlock), 0, q, batch, m->n_out, m->npad, m->out,
                        tab ? target : target + (size_t)a.first * m->n_out, m->head, 1.0f / batch,
                        1.0f / (batch * (float)nb), m->dout, m->doutt, m->loss_sum, tab, ctr);
-    backward(m, batch, q);
-    adam(m, a.lr, q, tab, ctr);
+    const AdamStep st = adam_step_args(m, a.lr, tab, ctr);
+    backward(m, batch, q, &st);
   };
   // Graph: one step captured with its changing arguments read from the
-  // device step table at *ctr, replayed once per step; its last node
-  // advances *ctr.  The table, the index and the per-epoch loss reset are
-  // written outside the graph.
+  // device step table at *ctr, replayed once per step; its weight-gradient
+  // launch advances *ctr (the Adam step after it reads the entry behind).
+  // The table, the index and the per-epoch loss reset are written outside
+  // the graph.
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   StepArgs* d_tab = nullptr;
@@ -1005,7 +1148,6 @@ int cdm_mlp_train(cdm_mlp* m, int n, int nmeas, const float* clean, const float*
       const long long step0 = m->adam_step;
       step_kernels(m->capture, steps[0], d_tab, d_ctr);
       m->adam_step = step0;  // the replays advance it below
-      hipLaunchKernelGGL(step_advance_kernel, dim3(1), dim3(64), 0, m->capture, d_ctr);
       const hipError_t ec = hipStreamEndCapture(m->capture, &graph);
       e = e == hipSuccess ? ec : e;
     }
