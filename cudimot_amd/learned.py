@@ -58,7 +58,14 @@ def train(model="NODDI_Bingham", n=2_000_000, epochs=20, batch=2048, lr=2e-3, sn
           stream=None):
     """Train a fresh network on n generated signals; returns (net, losses)."""
     t, d_t, clean, cfp = training_set(model, n, seed, stream=stream)
-    net = capi.Mlp(cfp.shape[0], head(model), width=WIDTH, depth=DEPTH, seed=seed)
+    return train_on(clean, d_t, model, epochs, batch, lr, snr, seed, stream)
+
+
+def train_on(clean, d_t, model="NODDI_Bingham", epochs=20, batch=2048, lr=2e-3, snr=(10.0, 50.0), seed=1234,
+             stream=None):
+    """Train a fresh network on a generated training set (training_set's
+    clean signals and truths); returns (net, losses)."""
+    net = capi.Mlp(clean.shape[1], head(model), width=WIDTH, depth=DEPTH, seed=seed)
     losses = net.train(clean, None, d_t, snr=snr, epochs=epochs, batch=batch, lr=lr, seed=seed + 1, stream=stream)
     return net, losses
 

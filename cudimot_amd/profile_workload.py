@@ -1,19 +1,40 @@
-"""Short workload for hardware-counter passes (rocprofv3 --pmc).
+"""Short workloads for profiler runs.
 
-LM (10 iterations) and MCMC (100 burn-in + 100 recorded jumps, a sample
-every 10) on 131,072 NODDI-Bingham voxels with 99 measurements at SNR 25,
-then a 512 MiB device-to-device copy whose FETCH_SIZE / WRITE_SIZE readings
-calibrate the traffic counters (its true traffic is 512 MiB each way).
+Default (hardware-counter passes, rocprofv3 --pmc): LM (10 iterations) and
+MCMC (100 burn-in + 100 recorded jumps, a sample every 10) on 131,072
+NODDI-Bingham voxels with 99 measurements at SNR 25, then a 512 MiB
+device-to-device copy whose FETCH_SIZE / WRITE_SIZE readings calibrate the
+traffic counters (its true traffic is 512 MiB each way).
 
-    python3 cudimot_amd/profile_workload.py
+--train: the NODDI-Bingham initialiser's training as the benchmark runs it
+(2,000,000 generated signals, 30 epochs, batches of 2,048), after a one-epoch
+warm-up on 65,536 signals; prints the training time.
+
+    python3 cudimot_amd/profile_workload.py [--train]
 """
 import os
 import sys
+import time
 
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
 
+def train():
+    import torch
+    from cudimot_amd import learned
+    learned.train("NODDI_Bingham", n=65536, epochs=1, seed=7)
+    t, d_t, clean, cfp = learned.training_set("NODDI_Bingham", 2_000_000, 1234)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    net, losses = learned.train_on(clean, d_t, "NODDI_Bingham", epochs=30, seed=1234)
+    torch.cuda.synchronize()
+    print("trained 2000000 signals x 30 epochs in %.3f s (loss %.4f -> %.4f)" % (time.time() - t0, losses[0],
+                                                                                 losses[-1]), flush=True)
+
+
 def main():
+    if "--train" in sys.argv[1:]:
+        return train()
     import torch
     from cudimot_amd import capi, synth
     n = 131072
