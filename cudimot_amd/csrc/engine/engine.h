@@ -65,6 +65,16 @@ constexpr int cdm_ntri = cdm::tri_size(NPARAMS);
 //                              CDM_VOXEL_CACHE_DEPENDS): an MCMC step in
 //                              another cache parameter recombines the kept
 //                              job values instead of recomputing them
+//   CDM_VOXEL_CACHE_PARTIAL    with the cache's jobs: after an MCMC step in
+//                              parameter `changed` alone, only the jobs
+//                              [first, first + count) of
+//                              Voxel_Cache_Job_Range(changed, first, count)
+//                              are recomputed (the others kept), and
+//                              Voxel_Cache_Update(npar, P, FixP, jobs, cur,
+//                              vc, changed) fills vc recomputing only the
+//                              entries that depend on `changed` and copying
+//                              the others from cur, with the arithmetic of
+//                              Voxel_Cache_Combine
 //   CDM_NUMERICAL_MASK b       derivatives the engine forms by central
 //                              differences (the reference's NUMERICAL step
 //                              rule) with per-perturbation voxel caches;
@@ -140,6 +150,40 @@ __device__ __forceinline__ void cdm_cache_jobs_g(TF* p, TF* fx, CDM_F64* jr, int
   }
 #pragma unroll
   for (int j = 0; j < cdm_vc_jobs; ++j) jr[j] = cdm::Group<L>::bcast(mine[j / L], j % L);
+}
+#endif
+
+#if defined(CDM_VOXEL_CACHE) && defined(CDM_VOXEL_CACHE_JOBS) && defined(CDM_VOXEL_CACHE_PARTIAL)
+// After a step in parameter `changed`: the jobs that depend on it, spread
+// over the group's lanes, the others copied from cur.
+template <int L, class TF>
+__device__ __forceinline__ void cdm_cache_jobs_part_g(TF* p, TF* fx, const CDM_F64* cur, CDM_F64* jr, int lane,
+                                                      int changed) {
+  int first = 0, count = 0;
+  Voxel_Cache_Job_Range(changed, first, count);
+  constexpr int rounds = (cdm_vc_jobs + L - 1) / L;
+  CDM_F64 mine[rounds];
+#pragma unroll
+  for (int r = 0; r < rounds; ++r) {
+    mine[r] = 0.0;
+    if (r * L < count) {
+      const int j = lane + r * L;
+      mine[r] = Voxel_Cache_Job(first + (j < count ? j : 0), cdm_np, p, fx);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < cdm_vc_jobs; ++j) {
+    const int k = j - first;
+    if (k >= 0 && k < count) {
+      CDM_F64 m = mine[0];
+#pragma unroll
+      for (int r = 1; r < rounds; ++r)
+        if (k / L == r) m = mine[r];
+      jr[j] = cdm::Group<L>::bcast(m, k % L);
+    } else {
+      jr[j] = cur[j];
+    }
+  }
 }
 #endif
 
@@ -1770,6 +1814,16 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
             cdm_tf pf[cdm_np];
 #pragma unroll
             for (int k = 0; k < cdm_np; ++k) pf[k] = (cdm_tf)p[k];
+#ifdef CDM_VOXEL_CACHE_PARTIAL
+            // only what parameter i moves: its jobs, its cache entries
+            if (jdep) {
+              cdm_cache_jobs_part_g<L>(pf, fx, jr_cur, jr_new, lane, i);
+            } else {
+#pragma unroll
+              for (int j = 0; j < cdm_vc_jobs; ++j) jr_new[j] = jr_cur[j];
+            }
+            Voxel_Cache_Update(cdm_np, pf, fx, jr_new, vc_cur, vc_new, i);
+#else
             if (jdep) {
               cdm_cache_jobs_g<L>(pf, fx, jr_new, lane);
             } else {
@@ -1777,6 +1831,7 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
               for (int j = 0; j < cdm_vc_jobs; ++j) jr_new[j] = jr_cur[j];
             }
             Voxel_Cache_Combine(cdm_np, pf, fx, jr_new, vc_new);
+#endif
 #endif
           } else {
             cdm_prepare_dg<L>(p, fx, vc_new, lane);

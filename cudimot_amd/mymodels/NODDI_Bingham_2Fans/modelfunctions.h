@@ -26,6 +26,7 @@
 #define CDM_MEAS_VOXEL_CACHE 2            // F1, F2 per measurement
 #define CDM_MEAS_VOXEL_CACHE_DEPENDS 0x3FFCu
 #define CDM_MEAS_VOXEL_CACHE_PARTIAL 1    // a step in one fan recomputes that fan
+#define CDM_VOXEL_CACHE_PARTIAL 1         // ... and only that fan's cache entries and jobs
 
 constexpr int b2f_fan = 13;  // cache values per fan
 
@@ -46,6 +47,43 @@ MACRO void Voxel_Cache_Combine(int npar, T* P, T* FixP, const CDM_F64* jobs, CDM
     c[2] = f.dperp1;
     c[3] = f.dperp2;
     bingham_fan_frame((CDM_F64)P[5 + o], (CDM_F64)P[6 + o], (CDM_F64)P[7 + o], c + 4);
+  }
+}
+
+// The jobs a step in `changed` moves: fan 1's five for kappa1, beta1, fan
+// 2's for kappa2, beta2.
+__device__ inline void Voxel_Cache_Job_Range(int changed, int& first, int& count) {
+  first = changed >= 8 ? 5 : 0;
+  count = (changed == 3 || changed == 4 || changed == 9 || changed == 10) ? 5 : 0;
+}
+
+// Voxel_Cache_Combine after a change of parameter `changed` alone: the
+// moved fan's normaliser and diffusivities (fintra, kappa, beta) or its
+// frame (an angle); every other entry copied from cur.
+MACRO void Voxel_Cache_Update(int npar, T* P, T* FixP, const CDM_F64* jobs, const CDM_F64* cur, CDM_F64* vc,
+                              int changed) {
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int o = 6 * k;
+    CDM_F64* c = vc + b2f_fan * k;
+    const CDM_F64* q = cur + b2f_fan * k;
+    if (changed >= 2 + o && changed < 5 + o) {
+      bingham_fan f;
+      bingham_fan_combine((CDM_F64)P[2 + o], jobs + 5 * k, f);
+      c[0] = f.lnc4;
+      c[1] = f.dpar;
+      c[2] = f.dperp1;
+      c[3] = f.dperp2;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) c[j] = q[j];
+    }
+    if (changed >= 5 + o && changed < 8 + o) {
+      bingham_fan_frame((CDM_F64)P[5 + o], (CDM_F64)P[6 + o], (CDM_F64)P[7 + o], c + 4);
+    } else {
+#pragma unroll
+      for (int j = 4; j < b2f_fan; ++j) c[j] = q[j];
+    }
   }
 }
 
