@@ -2,6 +2,10 @@
 //   S = S0 exp(-b g^T D g),  D = [[Dxx Dxy Dxz] [Dxy Dyy Dyz] [Dxz Dyz Dzz]]
 // P: S0, Dxx, Dxy, Dxz, Dyy, Dyz, Dzz.  CFP: g (3), b (1).
 // Follows DiffusionTensor/modelfunctions.h:20-114.
+// The compartment exponentials use texp_fast (in single precision the
+// hardware 2^y of y = x log2(e): relative error about |x| 2^-24 + 1 ulp,
+// below what the float rounding of the argument b g^T D g itself
+// brings); double-precision evaluations use the exact exp.
 
 MACRO void dti_design(const T* CFP, T* q) {
   // q[k] = d(b g^T D g)/dD_k for the six tensor elements in parameter order.
@@ -20,7 +24,7 @@ MACRO T Predicted_Signal(int npar, T* P, T* CFP, T* FixP) {
   T adc = (T)0;
 #pragma unroll
   for (int k = 0; k < 6; ++k) adc += q[k] * P[1 + k];
-  return P[0] * exp_gpu(-adc);
+  return P[0] * texp_fast(-adc);
 }
 
 MACRO bool ConstraintsMCMC(int npar, T* P) { return true; }
@@ -31,7 +35,7 @@ MACRO void Partial_Derivatives(int npar, T* P, T* CFP, T* FixP, T* derivatives) 
   T adc = (T)0;
 #pragma unroll
   for (int k = 0; k < 6; ++k) adc += q[k] * P[1 + k];
-  const T e = exp_gpu(-adc);
+  const T e = texp_fast(-adc);
   derivatives[0] = e;
 #pragma unroll
   for (int k = 0; k < 6; ++k) derivatives[1 + k] = -P[0] * q[k] * e;
