@@ -409,7 +409,6 @@ __global__ void decode_kernel(int n, int npad, const float* __restrict__ out, He
 struct AdamTensor {
   int nrow, ncol, rows_live, cols_live;
   float *w, *m, *v;
-  const float* g;
   bf16 *wbf, *wbft;
 };
 struct AdamArgs {
@@ -745,9 +744,9 @@ void wgrad_all(cdm_mlp* m, int rows, const AdamStep* step, hipStream_t s) {
     F.dzt = dzt;
     F.db = m->gb[l];
     if (step) {
-      F.tw = {m->rows[l], m->cols[l], m->rows_live[l], m->cols_live[l], m->w[l], m->mw[l], m->vw[l], m->gw[l],
-              m->wbf[l], m->wbft[l]};
-      F.tb = {m->rows[l], 1, m->rows_live[l], 1, m->b[l], m->mb[l], m->vb[l], m->gb[l], nullptr, nullptr};
+      F.tw = {m->rows[l], m->cols[l], m->rows_live[l], m->cols_live[l], m->w[l], m->mw[l], m->vw[l], m->wbf[l],
+              m->wbft[l]};
+      F.tb = {m->rows[l], 1, m->rows_live[l], 1, m->b[l], m->mb[l], m->vb[l], nullptr, nullptr};
     }
     fblocks += F.nb + units;
     off += (size_t)sp * units * cols;
