@@ -94,7 +94,7 @@ PLUGIN_MACROS = ["MODEL_H_INCLUDED", "NPARAMS", "NCFP", "NFIXP", "CDM_LM_JACOBIA
                  "CDM_VOXEL_CACHE_DEPENDS", "CDM_MEAS_CACHE", "CDM_NUMERICAL_MASK", "CDM_MEAS_VOXEL_CACHE",
                  "CDM_MEAS_VOXEL_CACHE_DEPENDS", "CDM_VOXEL_CACHE_JOBS", "CDM_VOXEL_CACHE_JOBS_DEPENDS",
                  "CDM_MEAS_VOXEL_CACHE_FILL", "CDM_MEAS_VOXEL_CACHE_PARTIAL", "CDM_VOXEL_CACHE_PARTIAL", "CDM_SIGNAL_PARTIALS", "CDM_WATSON_REFERENCE_FORMS",
-                 "CDM_LM_JACOBIAN_COLUMNS"]
+                 "CDM_LM_JACOBIAN_COLUMNS", "CDM_LM_ROW_PASSES"]
 
 
 def batch_unit_source(name, model_dirs):

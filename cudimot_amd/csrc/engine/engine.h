@@ -79,6 +79,11 @@ constexpr int cdm_ntri = cdm::tri_size(NPARAMS);
 //                              differences (the reference's NUMERICAL step
 //                              rule) with per-perturbation voxel caches;
 //                              Partial_Derivatives_Cached fills the others
+//   CDM_LM_ROW_PASSES          with CDM_NUMERICAL_MASK: 64-lane LM groups form
+//                              the row-wise Jacobian in two passes over the
+//                              measurements (the signal, the analytic entries
+//                              and the first half of the central differences,
+//                              then the rest), with half the probes live
 //   CDM_LM_JACOBIAN_COLUMNS    with CDM_NUMERICAL_MASK and the cache's jobs:
 //                              64-lane LM groups form the Jacobian column by
 //                              column (cdm_lm_jacobian_columns), holding two
@@ -760,18 +765,36 @@ struct cdm_jacobian_setup {
 #endif
   }
   __device__ __forceinline__ TJ eval(TJ* p, TJ* cm, TJ* fx, const CDM_F64* mc, TJ* d) {
+#ifdef CDM_NUMERICAL_MASK
+    return eval_part<CDM_NUMERICAL_MASK, true>(p, cm, fx, mc, d);
+#else
 #pragma unroll
     for (int i = 0; i < cdm_np; ++i) d[i] = (TJ)0;
-#ifdef CDM_NUMERICAL_MASK
-#ifdef CDM_SIGNAL_PARTIALS
-    const TJ s = Predicted_Signal_Partials_Cached(cdm_np, p, cm, fx, vc0, mc, d);
-#else
     const TJ s = cdm_signal(p, cm, fx, vc0, mc);
-    Partial_Derivatives_Cached(cdm_np, p, cm, fx, vc0, mc, d);
+    Partial_Derivatives(cdm_np, p, cm, fx, d);
+    return s;
 #endif
+  }
+#ifdef CDM_NUMERICAL_MASK
+  // eval restricted to the central differences of the parameters in PROBES
+  // (a subset of CDM_NUMERICAL_MASK), with the signal and the analytic
+  // entries when SIGNAL (0 is returned otherwise); each entry is eval's.
+  template <unsigned PROBES, bool SIGNAL>
+  __device__ __forceinline__ TJ eval_part(TJ* p, TJ* cm, TJ* fx, const CDM_F64* mc, TJ* d) {
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) d[i] = (TJ)0;
+    TJ s = (TJ)0;
+    if constexpr (SIGNAL) {
+#ifdef CDM_SIGNAL_PARTIALS
+      s = Predicted_Signal_Partials_Cached(cdm_np, p, cm, fx, vc0, mc, d);
+#else
+      s = cdm_signal(p, cm, fx, vc0, mc);
+      Partial_Derivatives_Cached(cdm_np, p, cm, fx, vc0, mc, d);
+#endif
+    }
 #pragma unroll
     for (int i = 0; i < cdm_np; ++i) {
-      if (!((CDM_NUMERICAL_MASK >> i) & 1u)) continue;
+      if (!((CDM_NUMERICAL_MASK >> i) & 1u) || !((PROBES >> i) & 1u)) continue;
       const bool dep = (cdm_vc_dep >> i) & 1u;
       const TJ keep = p[i];
       TJ up, dn;
@@ -793,13 +816,22 @@ struct cdm_jacobian_setup {
       d[i] = (up - dn) / h[i];
     }
     return s;
-#else
-    const TJ s = cdm_signal(p, cm, fx, vc0, mc);
-    Partial_Derivatives(cdm_np, p, cm, fx, d);
-    return s;
-#endif
   }
+#endif
 };
+
+// The first ceil(n/2) parameters of a mask of n (the first pass of
+// CDM_LM_ROW_PASSES).
+constexpr unsigned cdm_lm_first_half(unsigned mask) {
+  int keep = (__builtin_popcount(mask) + 1) / 2;
+  unsigned out = 0;
+  for (int b = 0; b < 32 && keep > 0; ++b)
+    if ((mask >> b) & 1u) {
+      out |= 1u << b;
+      --keep;
+    }
+  return out;
+}
 
 // A 64-lane LM group (one whole wave) sums J^T J and J^T r through LDS: each
 // lane stores the rows of its measurements, then lane e sums entry e (a
@@ -1166,6 +1198,31 @@ __device__ __forceinline__ void cdm_levmar_voxel(const cdm_lm_args& a, int marqu
         } else {
           cdm_jacobian_setup<cdm_jt> jac;
           jac.template init<L>(pj, fxj, lane);
+#if defined(CDM_LM_ROW_PASSES) && defined(CDM_NUMERICAL_MASK)
+          // two passes over the measurements: the signal, the analytic
+          // entries and the first half of the central differences, then the
+          // other half (fewer probes live at a time); the same entries
+          constexpr unsigned kFirst = cdm_lm_first_half(CDM_NUMERICAL_MASK);
+          constexpr unsigned kSecond = CDM_NUMERICAL_MASK & ~kFirst;
+          for (int m = lane; m < a.g.nmeas; m += L) {
+            cdm_jt* cm = rowj.at(s_cfp, m, a.g.cfp_ts);
+            cdm_jt d[cdm_np];
+            const cdm_jt r =
+                jac.template eval_part<kFirst, true>(pj, cm, fxj, cdm_mc_row(s_mc, m), d) - (cdm_jt)mv[m];
+#pragma unroll
+            for (int i = 0; i < cdm_np; ++i)
+              if (!((kSecond >> i) & 1u)) rows[m * (cdm_np + 1) + i] = (CDM_F64)(cdm_lm_acc)(d[i] * dtf[i]);
+            rows[m * (cdm_np + 1) + cdm_np] = (CDM_F64)(cdm_lm_acc)r;
+          }
+          for (int m = lane; m < a.g.nmeas; m += L) {
+            cdm_jt* cm = rowj.at(s_cfp, m, a.g.cfp_ts);
+            cdm_jt d[cdm_np];
+            (void)jac.template eval_part<kSecond, false>(pj, cm, fxj, cdm_mc_row(s_mc, m), d);
+#pragma unroll
+            for (int i = 0; i < cdm_np; ++i)
+              if ((kSecond >> i) & 1u) rows[m * (cdm_np + 1) + i] = (CDM_F64)(cdm_lm_acc)(d[i] * dtf[i]);
+          }
+#else
           for (int m = lane; m < a.g.nmeas; m += L) {
             cdm_jt* cm = rowj.at(s_cfp, m, a.g.cfp_ts);
             cdm_jt d[cdm_np];
@@ -1174,6 +1231,7 @@ __device__ __forceinline__ void cdm_levmar_voxel(const cdm_lm_args& a, int marqu
             for (int i = 0; i < cdm_np; ++i) rows[m * (cdm_np + 1) + i] = (CDM_F64)(cdm_lm_acc)(d[i] * dtf[i]);
             rows[m * (cdm_np + 1) + cdm_np] = (CDM_F64)(cdm_lm_acc)r;
           }
+#endif
         }
         cdm_wave_lds_sync();
         cdm_lds_normal_sums<L>(rows, sums, a.g.nmeas, lane);

@@ -26,6 +26,7 @@
 #define CDM_MEAS_VOXEL_CACHE_FILL 1         // cache row and signal together
 #define CDM_SIGNAL_PARTIALS 1               // signal and analytic entries together
 #define CDM_MEAS_CACHE 1                    // exp(-b Diso) per measurement
+#define CDM_LM_ROW_PASSES 2                 // 64-lane LM: the Jacobian rows in two passes
 
 MACRO void Voxel_Cache(int npar, T* P, T* FixP, CDM_F64* vc) {
   bingham_fan f;
