@@ -32,6 +32,11 @@ def test_batch_argument_errors_without_gpu():
              for name, _, _ in BATCH]
     with pytest.raises(capi.CdmError, match="needs tau_samples"):
         capi.batch_mcmc(items, nburn=10, njumps=10, sampleevery=5, rician=True)
+    # the per-item kernel threshold is process-wide; negative values are refused
+    with pytest.raises(capi.CdmError, match="blocks per CU"):
+        capi.batch_set_fill(-1)
+    capi.batch_set_fill(0)
+    capi.batch_set_fill(8)
 
 
 class _Fake:
