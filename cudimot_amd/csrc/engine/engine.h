@@ -364,20 +364,23 @@ struct cdm_gs_args {
   int* kept_out;            // [nvox] kept original index or -1 (nullable)
 };
 
-template <int L>
-__global__ void __launch_bounds__(cdm_block) cdm_gridsearch_kernel(cdm_gs_args a) {
-  extern __shared__ float s_cfp[];
-  const CDM_F64* s_mc = cdm_stage_cfp(a.g, s_cfp);
-  const int v = blockIdx.x * (cdm_block / L) + threadIdx.x / L;
-  const int lane = threadIdx.x % L;
-  if (v >= a.g.nvox) return;
-  const float* mv = a.g.meas + (size_t)v * a.g.nmeas;
-
+// One voxel's part of the search: the non-grid parameters' bounds test, the
+// fixed parameters and the FP32 screen of combinations [c0, c1) into a
+// sorted top-K list (strict <, so ties keep the earliest).
+struct cdm_gs_voxel {
   float base[cdm_np];
+  cdm_tf fx[cdm_fixp_max];
+  float best_c[CDM_GS_TOPK];
+  int best_i[CDM_GS_TOPK];
+};
+
+// Whether the voxel's non-grid parameters are inside the bounds (and the grid
+// is not empty); loads its parameters and fixed parameters.
+__device__ __forceinline__ bool cdm_gs_begin(const cdm_gs_args& a, int v, cdm_gs_voxel& w) {
   bool is_grid[cdm_np];
 #pragma unroll
   for (int i = 0; i < cdm_np; ++i) {
-    base[i] = a.params[(size_t)v * cdm_np + i];
+    w.base[i] = a.params[(size_t)v * cdm_np + i];
     is_grid[i] = false;
   }
   for (int k = 0; k < a.ngp; ++k) {
@@ -389,135 +392,339 @@ __global__ void __launch_bounds__(cdm_block) cdm_gridsearch_kernel(cdm_gs_args a
 #pragma unroll
   for (int i = 0; i < cdm_np; ++i) {
     if (is_grid[i]) continue;
-    const double x = base[i];
+    const double x = w.base[i];
     const int bt = a.c.bound_types[i];
     if ((bt == cdm_BMIN || bt == cdm_BMINMAX) && x < a.c.bmin[i]) ok = false;
     if ((bt == cdm_BMAX || bt == cdm_BMINMAX) && x > a.c.bmax[i]) ok = false;
   }
-  if (!ok) {
-    if (lane == 0 && a.kept_out) a.kept_out[v] = -1;
-    return;
-  }
-  cdm_tf fx[cdm_fixp_max];
-  cdm_load_fixp(a.g, v, fx);
-
-  // FP32 (MyType) screen with a sorted top-K list.
-  float best_c[CDM_GS_TOPK];
-  int best_i[CDM_GS_TOPK];
+  if (ok) cdm_load_fixp(a.g, v, w.fx);
 #pragma unroll
   for (int k = 0; k < CDM_GS_TOPK; ++k) {
-    best_c[k] = INFINITY;
-    best_i[k] = -1;
+    w.best_c[k] = INFINITY;
+    w.best_i[k] = -1;
   }
-  double sumsq = 0.0;
-  for (int m = lane; m < a.g.nmeas; m += L) sumsq += (double)mv[m] * (double)mv[m];
-  sumsq = cdm::Group<L>::sum(sumsq);
+  return ok;
+}
 
-  cdm_tf trial[cdm_np];
-  for (int c = 0; c < a.ncombs; ++c) {
+// Inserts (cost, index) into the sorted list; equal costs keep the earlier
+// entry first.
+__device__ __forceinline__ void cdm_gs_insert(cdm_gs_voxel& w, float cost, int c) {
+  if (cost < w.best_c[CDM_GS_TOPK - 1]) {
+    float cc = cost;
+    int ci = c;
 #pragma unroll
-    for (int i = 0; i < cdm_np; ++i) trial[i] = (cdm_tf)base[i];
+    for (int k = 0; k < CDM_GS_TOPK; ++k) {
+      if (cc < w.best_c[k]) {
+        const float tc = w.best_c[k];
+        const int ti = w.best_i[k];
+        w.best_c[k] = cc;
+        w.best_i[k] = ci;
+        cc = tc;
+        ci = ti;
+      }
+    }
+  }
+}
+
+template <int L>
+__device__ __forceinline__ void cdm_gs_screen(const cdm_gs_args& a, const float* s_cfp, const CDM_F64* s_mc,
+                                              const float* mv, cdm_gs_voxel& w, int c0, int c1, int lane) {
+  cdm_tf trial[cdm_np];
+  for (int c = c0; c < c1; ++c) {
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) trial[i] = (cdm_tf)w.base[i];
     for (int k = 0; k < a.ngp; ++k) {
       const float gv = a.grid_f[(size_t)c * a.ngp + k];
 #pragma unroll
       for (int i = 0; i < cdm_np; ++i)
         if (i == a.gparams[k]) trial[i] = (cdm_tf)gv;
     }
-    const float cost = (float)cdm_sse<L>(a.g, s_cfp, s_mc, mv, trial, fx, lane);
-    if (cost < best_c[CDM_GS_TOPK - 1]) {
-      // insertion keeps (cost, index) order; equal costs keep the earlier index first
-      float cc = cost;
-      int ci = c;
+    const float cost = (float)cdm_sse<L>(a.g, s_cfp, s_mc, mv, trial, w.fx, lane);
+    cdm_gs_insert(w, cost, c);
+  }
+}
+
+// The sum of squares of the voxel's measurements (the FP32 error band).
+template <int L>
+__device__ __forceinline__ double cdm_gs_sumsq(const cdm_gs_args& a, const float* mv, int lane) {
+  double sumsq = 0.0;
+  for (int m = lane; m < a.g.nmeas; m += L) sumsq += (double)mv[m] * (double)mv[m];
+  return cdm::Group<L>::sum(sumsq);
+}
+
+// Whether more than the K listed combinations may sit within the FP32 error
+// band of the best (rounding of the forward model and of the residual sum,
+// generous by a factor ~10): then every combination is rescored in FP64.
+__device__ __forceinline__ bool cdm_gs_crowded(const cdm_gs_voxel& w, double sumsq) {
+  const double b0 = w.best_c[0];
+  const double band = 4e-5 * sqrt(fmax(b0, 0.0) * sumsq) + 1e-4 * b0 + 1e-8 * sumsq;
+  return w.best_i[CDM_GS_TOPK - 1] >= 0 && (double)w.best_c[CDM_GS_TOPK - 1] <= b0 + band;
+}
+
+// FP64 cost of combination c.
+template <int L>
+__device__ __forceinline__ double cdm_gs_cost64(const cdm_gs_args& a, const float* s_cfp, const CDM_F64* s_mc,
+                                                const float* mv, const cdm_gs_voxel& w, int c, int lane) {
+  double trial_d[cdm_np];
+  double fx_d[cdm_fixp_max];
 #pragma unroll
-      for (int k = 0; k < CDM_GS_TOPK; ++k) {
-        if (cc < best_c[k]) {
-          const float tc = best_c[k];
-          const int ti = best_i[k];
-          best_c[k] = cc;
-          best_i[k] = ci;
-          cc = tc;
-          ci = ti;
-        }
-      }
+  for (int k = 0; k < cdm_fixp_max; ++k) fx_d[k] = (double)w.fx[k];
+#pragma unroll
+  for (int i = 0; i < cdm_np; ++i) trial_d[i] = (double)w.base[i];
+  for (int k = 0; k < a.ngp; ++k) {
+    const double gv = a.grid_d[(size_t)c * a.ngp + k];
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i)
+      if (i == a.gparams[k]) trial_d[i] = gv;
+  }
+  return cdm_sse<L>(a.g, s_cfp, s_mc, mv, trial_d, fx_d, lane);
+}
+
+// The FP64 minimum over combinations [c0, c1) (the earliest on ties).
+template <int L>
+__device__ __forceinline__ void cdm_gs_rescan(const cdm_gs_args& a, const float* s_cfp, const CDM_F64* s_mc,
+                                              const float* mv, const cdm_gs_voxel& w, int c0, int c1, int lane,
+                                              double& bestd, int& bestd_i) {
+  for (int c = c0; c < c1; ++c) {
+    const double cd = cdm_gs_cost64<L>(a, s_cfp, s_mc, mv, w, c, lane);
+    if (cd < bestd) {
+      bestd = cd;
+      bestd_i = c;
     }
   }
+}
 
-  int chosen = best_i[0];
-  double chosen_cost = best_c[0];
+// The FP64 rescore of the listed candidates, in ascending enumeration order
+// (the earliest on ties).
+template <int L>
+__device__ __forceinline__ void cdm_gs_rescore_list(const cdm_gs_args& a, const float* s_cfp, const CDM_F64* s_mc,
+                                                    const float* mv, const cdm_gs_voxel& w, int lane, double& bestd,
+                                                    int& bestd_i) {
+  int cand[CDM_GS_TOPK];
+#pragma unroll
+  for (int k = 0; k < CDM_GS_TOPK; ++k) cand[k] = w.best_i[k] < 0 ? INT_MAX : w.best_i[k];
+#pragma unroll
+  for (int i = 0; i < CDM_GS_TOPK; ++i)
+#pragma unroll
+    for (int k = 0; k + 1 < CDM_GS_TOPK - i; ++k)
+      if (cand[k] > cand[k + 1]) {
+        const int tmp = cand[k];
+        cand[k] = cand[k + 1];
+        cand[k + 1] = tmp;
+      }
+  for (int q = 0; q < CDM_GS_TOPK; ++q) {
+    int c = cand[0];
+#pragma unroll
+    for (int k = 0; k < CDM_GS_TOPK; ++k)
+      if (k == q) c = cand[k];
+    if (c == INT_MAX) break;
+    const double cd = cdm_gs_cost64<L>(a, s_cfp, s_mc, mv, w, c, lane);
+    if (cd < bestd) {
+      bestd = cd;
+      bestd_i = c;
+    }
+  }
+}
+
+// The outputs for the chosen combination (-1: none): the grid parameters,
+// the kept index, and the --debug record.
+__device__ __forceinline__ void cdm_gs_write(const cdm_gs_args& a, const cdm_gs_voxel& w, int v, int lane, int chosen,
+                                             double chosen_cost) {
+  if (lane != 0) return;
+  if (chosen < 0) {
+    if (a.kept_out) a.kept_out[v] = -1;
+    return;
+  }
+  for (int k = 0; k < a.ngp; ++k) a.params[(size_t)v * cdm_np + a.gparams[k]] = a.grid_f[(size_t)chosen * a.ngp + k];
+  if (a.kept_out) a.kept_out[v] = a.comb_id[chosen];
+  if (a.c.dbg && v == a.c.dbg_voxel) {
+    // the kept combination, its cost (the FP64 rescore's, or the screen's
+    // when the plug-in is double), the valid combinations, the parameters
+    double pk[cdm_np];
+#pragma unroll
+    for (int i = 0; i < cdm_np; ++i) pk[i] = (double)w.base[i];
+    for (int k = 0; k < a.ngp; ++k) {
+#pragma unroll
+      for (int i = 0; i < cdm_np; ++i)
+        if (i == a.gparams[k]) pk[i] = (double)a.grid_f[(size_t)chosen * a.ngp + k];
+    }
+    cdm_dbg_record(a.c, 0, 0, (double)a.comb_id[chosen], chosen_cost, (double)a.ncombs, 0.0, pk);
+  }
+}
+
+// The rest of the search from the screened list: the FP64 rescore of the
+// candidates (of every combination when crowded), then the outputs.
+template <int L>
+__device__ __forceinline__ void cdm_gs_finish(const cdm_gs_args& a, const float* s_cfp, const CDM_F64* s_mc,
+                                              const float* mv, cdm_gs_voxel& w, int v, int lane) {
+  const double sumsq = cdm_gs_sumsq<L>(a, mv, lane);
+  int chosen = w.best_i[0];
+  double chosen_cost = w.best_c[0];
   if constexpr (sizeof(cdm_tf) != sizeof(double)) {
-    // Error band of the FP32 cost (rounding of the forward model and of the
-    // residual sum), generous by a factor ~10.
-    const double b0 = best_c[0];
-    const double band = 4e-5 * sqrt(fmax(b0, 0.0) * sumsq) + 1e-4 * b0 + 1e-8 * sumsq;
-    const bool crowded = best_i[CDM_GS_TOPK - 1] >= 0 && (double)best_c[CDM_GS_TOPK - 1] <= b0 + band;
-    // Candidates in ascending enumeration order (empty slots last).
-    int cand[CDM_GS_TOPK];
-#pragma unroll
-    for (int k = 0; k < CDM_GS_TOPK; ++k) cand[k] = best_i[k] < 0 ? INT_MAX : best_i[k];
-#pragma unroll
-    for (int i = 0; i < CDM_GS_TOPK; ++i)
-#pragma unroll
-      for (int k = 0; k + 1 < CDM_GS_TOPK - i; ++k)
-        if (cand[k] > cand[k + 1]) {
-          const int tmp = cand[k];
-          cand[k] = cand[k + 1];
-          cand[k + 1] = tmp;
-        }
-    double trial_d[cdm_np];
-    double fx_d[cdm_fixp_max];
-#pragma unroll
-    for (int k = 0; k < cdm_fixp_max; ++k) fx_d[k] = (double)fx[k];
     double bestd = INFINITY;
     int bestd_i = -1;
-    const int n_rescore = crowded ? a.ncombs : CDM_GS_TOPK;
-    for (int q = 0; q < n_rescore; ++q) {
-      int c = q;
-      if (!crowded) {
-#pragma unroll
-        for (int k = 0; k < CDM_GS_TOPK; ++k)
-          if (k == q) c = cand[k];
-        if (c == INT_MAX) break;
-      }
-#pragma unroll
-      for (int i = 0; i < cdm_np; ++i) trial_d[i] = (double)base[i];
-      for (int k = 0; k < a.ngp; ++k) {
-        const double gv = a.grid_d[(size_t)c * a.ngp + k];
-#pragma unroll
-        for (int i = 0; i < cdm_np; ++i)
-          if (i == a.gparams[k]) trial_d[i] = gv;
-      }
-      const double cd = cdm_sse<L>(a.g, s_cfp, s_mc, mv, trial_d, fx_d, lane);
-      if (cd < bestd) {
-        bestd = cd;
-        bestd_i = c;
-      }
-    }
+    if (cdm_gs_crowded(w, sumsq))
+      cdm_gs_rescan<L>(a, s_cfp, s_mc, mv, w, 0, a.ncombs, lane, bestd, bestd_i);
+    else
+      cdm_gs_rescore_list<L>(a, s_cfp, s_mc, mv, w, lane, bestd, bestd_i);
     chosen = bestd_i;
     chosen_cost = bestd;
   }
-  if (chosen < 0) {
+  cdm_gs_write(a, w, v, lane, chosen, chosen_cost);
+}
+
+template <int L>
+__global__ void __launch_bounds__(cdm_block) cdm_gridsearch_kernel(cdm_gs_args a) {
+  extern __shared__ float s_cfp[];
+  const CDM_F64* s_mc = cdm_stage_cfp(a.g, s_cfp);
+  const int v = blockIdx.x * (cdm_block / L) + threadIdx.x / L;
+  const int lane = threadIdx.x % L;
+  if (v >= a.g.nvox) return;
+  const float* mv = a.g.meas + (size_t)v * a.g.nmeas;
+  cdm_gs_voxel w;
+  if (!cdm_gs_begin(a, v, w)) {
     if (lane == 0 && a.kept_out) a.kept_out[v] = -1;
     return;
   }
+  cdm_gs_screen<L>(a, s_cfp, s_mc, mv, w, 0, a.ncombs, lane);
+  cdm_gs_finish<L>(a, s_cfp, s_mc, mv, w, v, lane);
+}
+
+// Small volumes (fewer voxel groups than the GPU holds): the screen split
+// over S slices of the combinations, block (x, z) screening slice z of its
+// voxels into split_c / split_i [nvox][S][K]; cdm_gridsearch_merge_kernel
+// merges each voxel's S lists in slice order with the same insertion, which
+// leaves the K smallest (cost, index) pairs, the sequential screen's list.
+// A voxel that is not crowded is finished there; a crowded one is marked in
+// crowd[v] and its FP64 rescan split the same way
+// (cdm_gridsearch_rescan_kernel: slice z's minimum, the earliest on ties, into
+// rescan_c / rescan_i [nvox][S]), then cdm_gridsearch_pick_kernel takes the
+// minimum over the slices in order (strict <: the earliest on ties), the
+// sequential rescan's choice.
+struct cdm_gs_split {
+  int S;
+  float* list_c;     // [nvox][S][K]
+  int* list_i;       // [nvox][S][K]
+  int* crowd;        // [nvox]
+  double* rescan_c;  // [nvox][S]
+  int* rescan_i;     // [nvox][S]
+};
+
+__device__ __forceinline__ void cdm_gs_slice(const cdm_gs_args& a, int S, int z, int& c0, int& c1) {
+  const int per = (a.ncombs + S - 1) / S;
+  c0 = min(a.ncombs, z * per);
+  c1 = min(a.ncombs, c0 + per);
+}
+
+template <int L>
+__global__ void __launch_bounds__(cdm_block) cdm_gridsearch_split_kernel(cdm_gs_args a, cdm_gs_split sp) {
+  extern __shared__ float s_cfp[];
+  const CDM_F64* s_mc = cdm_stage_cfp(a.g, s_cfp);
+  const int v = blockIdx.x * (cdm_block / L) + threadIdx.x / L;
+  const int lane = threadIdx.x % L;
+  if (v >= a.g.nvox) return;
+  const int z = blockIdx.y;
+  const float* mv = a.g.meas + (size_t)v * a.g.nmeas;
+  cdm_gs_voxel w;
+  if (cdm_gs_begin(a, v, w)) {
+    int c0, c1;
+    cdm_gs_slice(a, sp.S, z, c0, c1);
+    cdm_gs_screen<L>(a, s_cfp, s_mc, mv, w, c0, c1, lane);
+  }
   if (lane == 0) {
-    for (int k = 0; k < a.ngp; ++k) a.params[(size_t)v * cdm_np + a.gparams[k]] = a.grid_f[(size_t)chosen * a.ngp + k];
-    if (a.kept_out) a.kept_out[v] = a.comb_id[chosen];
-    if (a.c.dbg && v == a.c.dbg_voxel) {
-      // the kept combination, its cost (the FP64 rescore's, or the screen's
-      // when the plug-in is double), the valid combinations, the parameters
-      double pk[cdm_np];
+    const size_t o = ((size_t)v * sp.S + z) * CDM_GS_TOPK;
 #pragma unroll
-      for (int i = 0; i < cdm_np; ++i) pk[i] = (double)base[i];
-      for (int k = 0; k < a.ngp; ++k) {
-#pragma unroll
-        for (int i = 0; i < cdm_np; ++i)
-          if (i == a.gparams[k]) pk[i] = (double)a.grid_f[(size_t)chosen * a.ngp + k];
-      }
-      double kc = best_c[0];
-      if constexpr (sizeof(cdm_tf) != sizeof(double)) kc = chosen_cost;
-      cdm_dbg_record(a.c, 0, 0, (double)a.comb_id[chosen], kc, (double)a.ncombs, 0.0, pk);
+    for (int k = 0; k < CDM_GS_TOPK; ++k) {
+      sp.list_c[o + k] = w.best_c[k];
+      sp.list_i[o + k] = w.best_i[k];
     }
   }
+}
+
+// The voxel's merged screen list.
+__device__ __forceinline__ void cdm_gs_merged(const cdm_gs_split& sp, int v, cdm_gs_voxel& w) {
+  for (int z = 0; z < sp.S; ++z) {
+    const size_t o = ((size_t)v * sp.S + z) * CDM_GS_TOPK;
+#pragma unroll
+    for (int k = 0; k < CDM_GS_TOPK; ++k) {
+      const int ci = sp.list_i[o + k];
+      if (ci >= 0) cdm_gs_insert(w, sp.list_c[o + k], ci);
+    }
+  }
+}
+
+template <int L>
+__global__ void __launch_bounds__(cdm_block) cdm_gridsearch_merge_kernel(cdm_gs_args a, cdm_gs_split sp) {
+  extern __shared__ float s_cfp[];
+  const CDM_F64* s_mc = cdm_stage_cfp(a.g, s_cfp);
+  const int v = blockIdx.x * (cdm_block / L) + threadIdx.x / L;
+  const int lane = threadIdx.x % L;
+  if (v >= a.g.nvox) return;
+  const float* mv = a.g.meas + (size_t)v * a.g.nmeas;
+  cdm_gs_voxel w;
+  if (!cdm_gs_begin(a, v, w)) {
+    if (lane == 0) sp.crowd[v] = 0;
+    cdm_gs_write(a, w, v, lane, -1, 0.0);
+    return;
+  }
+  cdm_gs_merged(sp, v, w);
+  if constexpr (sizeof(cdm_tf) != sizeof(double)) {
+    const double sumsq = cdm_gs_sumsq<L>(a, mv, lane);
+    const bool crowded = cdm_gs_crowded(w, sumsq);
+    if (lane == 0) sp.crowd[v] = crowded ? 1 : 0;
+    if (crowded) return;
+    double bestd = INFINITY;
+    int bestd_i = -1;
+    cdm_gs_rescore_list<L>(a, s_cfp, s_mc, mv, w, lane, bestd, bestd_i);
+    cdm_gs_write(a, w, v, lane, bestd_i, bestd);
+  } else {
+    if (lane == 0) sp.crowd[v] = 0;
+    cdm_gs_write(a, w, v, lane, w.best_i[0], w.best_c[0]);
+  }
+}
+
+template <int L>
+__global__ void __launch_bounds__(cdm_block) cdm_gridsearch_rescan_kernel(cdm_gs_args a, cdm_gs_split sp) {
+  extern __shared__ float s_cfp[];
+  const CDM_F64* s_mc = cdm_stage_cfp(a.g, s_cfp);
+  const int v = blockIdx.x * (cdm_block / L) + threadIdx.x / L;
+  const int lane = threadIdx.x % L;
+  if (v >= a.g.nvox) return;
+  if (!sp.crowd[v]) return;
+  const int z = blockIdx.y;
+  const float* mv = a.g.meas + (size_t)v * a.g.nmeas;
+  cdm_gs_voxel w;
+  (void)cdm_gs_begin(a, v, w);
+  int c0, c1;
+  cdm_gs_slice(a, sp.S, z, c0, c1);
+  double bestd = INFINITY;
+  int bestd_i = -1;
+  cdm_gs_rescan<L>(a, s_cfp, s_mc, mv, w, c0, c1, lane, bestd, bestd_i);
+  if (lane == 0) {
+    sp.rescan_c[(size_t)v * sp.S + z] = bestd;
+    sp.rescan_i[(size_t)v * sp.S + z] = bestd_i;
+  }
+}
+
+template <int L>
+__global__ void __launch_bounds__(cdm_block) cdm_gridsearch_pick_kernel(cdm_gs_args a, cdm_gs_split sp) {
+  const int v = blockIdx.x * (cdm_block / L) + threadIdx.x / L;
+  const int lane = threadIdx.x % L;
+  if (v >= a.g.nvox) return;
+  if (!sp.crowd[v]) return;
+  cdm_gs_voxel w;
+  (void)cdm_gs_begin(a, v, w);
+  double bestd = INFINITY;
+  int bestd_i = -1;
+  for (int z = 0; z < sp.S; ++z) {
+    const double cd = sp.rescan_c[(size_t)v * sp.S + z];
+    const int ci = sp.rescan_i[(size_t)v * sp.S + z];
+    if (ci >= 0 && cd < bestd) {
+      bestd = cd;
+      bestd_i = ci;
+    }
+  }
+  cdm_gs_write(a, w, v, lane, bestd_i, bestd);
 }
 
 // ------------------------------------------------------ Levenberg-Marquardt

@@ -20,9 +20,25 @@
     default: return hipErrorInvalidValue;  \
   }
 
+template <int L>
+static void op_gridsearch_split(const cdm::Geom& g, const cdm_gs_args& a, const cdm_gs_split& sp, hipStream_t s) {
+  const dim3 grid = op_grid(g.nvox, L), grid_s(grid.x, (unsigned)sp.S);
+  hipLaunchKernelGGL(cdm_gridsearch_split_kernel<L>, grid_s, dim3(cdm_block), op_lds(g), s, a, sp);
+  hipLaunchKernelGGL(cdm_gridsearch_merge_kernel<L>, grid, dim3(cdm_block), op_lds(g), s, a, sp);
+  if constexpr (sizeof(cdm_tf) != sizeof(double)) {
+    hipLaunchKernelGGL(cdm_gridsearch_rescan_kernel<L>, grid_s, dim3(cdm_block), op_lds(g), s, a, sp);
+    hipLaunchKernelGGL(cdm_gridsearch_pick_kernel<L>, grid, dim3(cdm_block), 0, s, a, sp);
+  }
+}
+
+// S > 1: the search split over S slices of the combinations, in the scratch
+// `split` (cdm::gs_split_bytes(nvox, S) bytes): the screen
+// (cdm_gridsearch_split_kernel), the merge that finishes the voxels that are
+// not crowded (cdm_gridsearch_merge_kernel), the crowded voxels' FP64 rescan
+// (cdm_gridsearch_rescan_kernel) and their choice (cdm_gridsearch_pick_kernel).
 static hipError_t op_gridsearch(int L, const cdm::Geom& g, float* params, int ngp, const int* gparams, int ncombs,
                                 const float* grid_f, const double* grid_d, const int* comb_id, const cdm::Consts& c,
-                                int* kept, hipStream_t s) {
+                                int* kept, int S, void* split, hipStream_t s) {
   if (g.nvox <= 0) return hipSuccess;
   if (ngp > cdm_np || g.cfp_ts > cdm_cfp_max || g.fixp_ts > cdm_fixp_max) return hipErrorInvalidValue;
   cdm_gs_args a;
@@ -36,6 +52,18 @@ static hipError_t op_gridsearch(int L, const cdm::Geom& g, float* params, int ng
   a.comb_id = comb_id;
   a.c = op_consts(c);
   a.kept_out = kept;
+  if (S > 1 && split) {
+    const size_t n = (size_t)g.nvox * S;
+    cdm_gs_split sp;
+    sp.S = S;
+    sp.rescan_c = (double*)split;
+    sp.list_c = (float*)(sp.rescan_c + n);
+    sp.list_i = (int*)(sp.list_c + n * CDM_GS_TOPK);
+    sp.rescan_i = sp.list_i + n * CDM_GS_TOPK;
+    sp.crowd = sp.rescan_i + n;
+    CDM_DISPATCH_L(L, op_gridsearch_split<LG>(g, a, sp, s));
+    return hipGetLastError();
+  }
   CDM_DISPATCH_L(L, hipLaunchKernelGGL(cdm_gridsearch_kernel<LG>, op_grid(g.nvox, LG), dim3(cdm_block), op_lds(g),
                                        s, a));
   return hipGetLastError();

@@ -68,6 +68,10 @@ struct cdm_model {
   size_t state_bytes = 0;
   // LM work-queue counter (one int; calls on one handle are serialised)
   int* d_queue = nullptr;
+  // scratch of a GridSearch split over slices of the combinations
+  // (cdm::gs_split_bytes), grown as needed
+  void* d_gs_split = nullptr;
+  size_t gs_split_bytes = 0;
   // MCMC phases recorded into HIP graphs (jump blocks with use_graph), one
   // per phase, and the private stream they are captured on
   GraphCache mcmc_graphs[2];
@@ -476,6 +480,7 @@ void cdm_model_close(cdm_model* m) {
   free_grid(m);
   if (m->d_state) (void)hipFree(m->d_state);
   if (m->d_queue) (void)hipFree(m->d_queue);
+  if (m->d_gs_split) (void)hipFree(m->d_gs_split);
   for (GraphCache& g : m->mcmc_graphs) g.clear();
   if (m->capture_stream) (void)hipStreamDestroy(m->capture_stream);
   if (m->last_done) (void)hipEventDestroy(m->last_done);
@@ -638,6 +643,25 @@ int cdm_model_set_mcmc_blocks(cdm_model* m, int jumps_per_launch, int use_graph)
   return 0;
 }
 
+// Slices of the combinations for a GridSearch whose voxel groups alone
+// leave the GPU mostly idle: enough (at most 16, at least 64 combinations
+// each) for about 8 blocks per CU; 1 for volumes that fill it.
+int gs_splits(int L, int nvox, int ncombs) {
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) {
+    (void)hipGetLastError();
+    return 1;
+  }
+  const long long per_block = 256 / L;
+  const long long blocks = ((long long)nvox + per_block - 1) / per_block;
+  const long long target = 8LL * ncu;
+  if (blocks >= target) return 1;
+  long long S = std::min<long long>(16, (target + blocks - 1) / blocks);
+  S = std::min<long long>(S, std::max(1, ncombs / 64));
+  return (int)std::max<long long>(1, S);
+}
+
 int cdm_gridsearch(cdm_model* m, int nvox, int nmeas, const float* meas, float* params, const float* cfp,
                    const float* fixp, int* kept, void* stream) {
   int r = check_geom(m, nvox, nmeas, meas, cfp, fixp, true);
@@ -647,12 +671,34 @@ int cdm_gridsearch(cdm_model* m, int nvox, int nmeas, const float* meas, float* 
   if (nvox == 0) return 0;
   const cdm::Geom g = make_geom(m, nvox, nmeas, meas, cfp, fixp);
   const cdm::Consts c = make_consts(m);
-  hipError_t e = debug_reset(m, 0, (hipStream_t)stream);
-  if (e != hipSuccess) return hip_fail(e, "GridSearch --debug records");
-  e = m->ops->gridsearch(group_for(m, cdm::kGridSearch, nmeas, nvox), g, params, (int)m->spec.grid_params.size(),
-                         m->spec.grid_params.data(), m->ncombs_valid, m->d_grid_f, m->d_grid_d, m->d_comb_id, c, kept,
-                         (hipStream_t)stream);
-  return e == hipSuccess ? 0 : hip_fail(e, "GridSearch launch");
+  hipStream_t s = (hipStream_t)stream;
+  const int L = group_for(m, cdm::kGridSearch, nmeas, nvox);
+  const int S = gs_splits(L, nvox, m->ncombs_valid);
+  hipError_t e = hipSuccess;
+  if (S > 1) {
+    // the lists belong to the handle: order after its last launch
+    const size_t bytes = cdm::gs_split_bytes(nvox, S);
+    if (m->gs_split_bytes < bytes) {
+      if ((e = order_after_last(m, s)) != hipSuccess) return hip_fail(e, "GridSearch stream order");
+      if (m->has_last && (e = hipStreamSynchronize(m->last_stream)) != hipSuccess)
+        return hip_fail(e, "GridSearch scratch");
+      if (m->d_gs_split) (void)hipFree(m->d_gs_split);
+      m->d_gs_split = nullptr;
+      m->gs_split_bytes = 0;
+      if ((e = hipMalloc(&m->d_gs_split, bytes)) != hipSuccess) {
+        m->d_gs_split = nullptr;
+        return hip_fail(e, "GridSearch scratch");
+      }
+      m->gs_split_bytes = bytes;
+    }
+    if ((e = order_after_last(m, s)) != hipSuccess) return hip_fail(e, "GridSearch stream order");
+  }
+  if ((e = debug_reset(m, 0, s)) != hipSuccess) return hip_fail(e, "GridSearch --debug records");
+  e = m->ops->gridsearch(L, g, params, (int)m->spec.grid_params.size(), m->spec.grid_params.data(), m->ncombs_valid,
+                         m->d_grid_f, m->d_grid_d, m->d_comb_id, c, kept, S, S > 1 ? m->d_gs_split : nullptr, s);
+  if (e != hipSuccess) return hip_fail(e, "GridSearch launch");
+  if (S > 1) (void)note_last(m, s);
+  return 0;
 }
 
 int cdm_levmar(cdm_model* m, int nvox, int nmeas, const float* meas, float* params, const float* cfp,

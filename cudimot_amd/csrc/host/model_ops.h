@@ -45,6 +45,14 @@ struct McmcLaunch {
 
 enum Method { kGridSearch = 0, kLevMar = 1, kMcmc = 2, kPredicted = 3, kBicAic = 4 };
 
+// Scratch of a GridSearch split over S slices (per voxel and slice: an FP64
+// rescan minimum, 4 screened costs and indices, the rescan index; per voxel
+// a crowded flag).
+inline size_t gs_split_bytes(int nvox, int S) {
+  const size_t n = (size_t)nvox * S;
+  return n * (sizeof(double) + 4 * (sizeof(float) + sizeof(int)) + sizeof(int)) + (size_t)nvox * sizeof(int);
+}
+
 struct ModelOps {
   const char* name;
   int nparams;
@@ -56,9 +64,11 @@ struct ModelOps {
   size_t chain_bytes;
   size_t mvc_bytes_per_meas;  // per-measurement cache of one state (0: none)
   int cfp_max, fixp_max;
+  // S > 1: the search in S slices of the combinations, with gs_split_bytes
+  // of scratch at `split`
   hipError_t (*gridsearch)(int L, const Geom& g, float* params, int ngp, const int* gparams, int ncombs,
                            const float* grid_f, const double* grid_d, const int* comb_id, const Consts& c,
-                           int* kept, hipStream_t s);
+                           int* kept, int S, void* split, hipStream_t s);
   // W: register budget, in resident 256-thread blocks per CU (1, 2 or 4)
   // queue: device int for the work-queue kernel (null: static voxel mapping);
   // ncu: compute units of the device
