@@ -15,11 +15,11 @@
  *   tau      [nvox][nsamp]            MCMC.cu:624
  * `stream` is a hipStream_t (NULL: the null stream); calls are asynchronous
  * on it.  A model handle is not safe for concurrent use from several host
- * threads.  Its LM work-queue counter and MCMC chain state belong to the
- * handle: an LM or MCMC call on another stream than the handle's previous
- * one waits on the device for that call to finish (outside stream capture;
- * a capturing caller orders its streams itself).  Use one handle per stream
- * for concurrent fits.
+ * threads.  Its LM work-queue counter, MCMC chain state and GridSearch
+ * scratch (small volumes) belong to the handle: such a call on another
+ * stream than the handle's previous one waits on the device for that call
+ * to finish (outside stream capture; a capturing caller orders its streams
+ * itself).  Use one handle per stream for concurrent fits.
  */
 #ifndef CUDIMOT_AMD_H
 #define CUDIMOT_AMD_H
@@ -114,7 +114,10 @@ int cdm_model_graph_stats(const cdm_model* m, long long* captures, long long* re
 /* GridSearch<T>::run (GridSearch.cu:291-317; kernel :97-248).  Updates the
  * grid parameters of `params` to the combination of least squared error
  * (the FP64 argmin, earliest combination on ties).  kept (nullable,
- * device int[nvox]) receives the kept combination index, -1 if none. */
+ * device int[nvox]) receives the kept combination index, -1 if none.
+ * Volumes too small to fill the GPU search slices of the combinations in
+ * parallel, with the same result (scratch allocated on the handle at the
+ * first such call of a larger size). */
 int cdm_gridsearch(cdm_model* m, int nvox, int nmeas, const float* meas, float* params, const float* cfp,
                    const float* fixp, int* kept, void* stream);
 
