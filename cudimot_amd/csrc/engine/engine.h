@@ -1679,8 +1679,16 @@ struct cdm_chain {  // FP64 chain state per voxel
 // (one slot per voxel group of the block) rather than registers: a step
 // indexes it by the proposed parameter, and registers are the kernel's
 // occupancy limit.  All lanes of the group write identical values.
+// A large voxel cache (16 values or more) keeps the current state's copy,
+// and its cache jobs, in the voxel's LDS slot rather than in registers: a
+// step holds the proposal's cache in registers, and the current one is read
+// to select or copy entries and written on acceptance.
+constexpr bool cdm_vc_in_lds = cdm_vc >= 16;
+
 struct cdm_vslot {
   double p[cdm_np], prior[cdm_np], sd[cdm_np];
+  CDM_F64 vc[cdm_vc_in_lds ? cdm_vc : 1];
+  CDM_F64 jr[cdm_vc_in_lds && cdm_vc_jobs > 0 ? cdm_vc_jobs : 1];
   int nacc[cdm_np], nrej[cdm_np];
 };
 
@@ -1964,8 +1972,10 @@ __device__ __forceinline__ void cdm_mcmc_body(const cdm_mcmc_args& a, int block)
 #else
   constexpr bool job_split = false;
 #endif
-  CDM_F64 vc_cur[cdm_vc];
-  CDM_F64 jr_cur[job_split ? cdm_vc_jobs : 1];
+  CDM_F64 vc_reg[cdm_vc_in_lds ? 1 : cdm_vc];
+  CDM_F64 jr_reg[!cdm_vc_in_lds && job_split ? cdm_vc_jobs : 1];
+  CDM_F64* vc_cur = cdm_vc_in_lds ? vs->vc : vc_reg;
+  CDM_F64* jr_cur = cdm_vc_in_lds ? vs->jr : jr_reg;
   if constexpr (job_split) {
 #if defined(CDM_VOXEL_CACHE) && defined(CDM_VOXEL_CACHE_JOBS)
     cdm_tf pf[cdm_np];
