@@ -134,6 +134,39 @@ def test_gridsearch_argmin_exact(torch_cuda, model, lanes):
     np.testing.assert_allclose(H.host(dp), p_ref, rtol=1e-6, atol=1e-7)
 
 
+@pytest.mark.parametrize("model", ["NODDI_Bingham", "Ball_2_Sticks"])
+def test_gridsearch_split_equals_single_scan(torch_cuda, model):
+    """A volume too small to fill the GPU searches slices of the grid in
+    parallel (split screen, merge, split FP64 rescan of crowded voxels); the
+    same voxels inside a volume that fills the GPU take the single scan.  At
+    the same voxel-group size both keep the same combination and parameters,
+    bit for bit."""
+    from cudimot_amd import capi
+    o = bind.OracleModel(model)
+    m = capi.Model(model)
+    m.set_group(capi.GRIDSEARCH, 4)
+    m.load_grid(GRIDS[model], is_text=True)
+    spec = m.spec()
+    n, copies = 300, 800  # 300 voxels: 5 blocks of 4-lane groups; 240,000: 3,750 blocks
+    cfp, t, fx, noisy = H.volume(o, model, n, seed=23)
+    p0 = _start(model, spec, noisy, cfp, t)
+    if model.startswith("NODDI"):
+        p0 = _clip_noddi(model, t.copy())
+    small_p = H.dev(p0)
+    small_k = torch_cuda.full((n,), -7, dtype=torch_cuda.int32, device="cuda")
+    m.gridsearch(H.dev(noisy), small_p, H.dev(cfp), H.dev(fx), kept=small_k)
+    big_fx = None if fx is None else np.tile(fx, (copies, 1))
+    big_p = H.dev(np.tile(p0, (copies, 1)))
+    big_k = torch_cuda.full((n * copies,), -7, dtype=torch_cuda.int32, device="cuda")
+    m.gridsearch(H.dev(np.tile(noisy, (copies, 1))), big_p, H.dev(cfp), H.dev(big_fx), kept=big_k)
+    capi.sync()
+    ks, kb = small_k.cpu().numpy(), big_k.cpu().numpy()
+    assert (ks >= 0).mean() > 0.9
+    for c in (0, copies // 2, copies - 1):
+        np.testing.assert_array_equal(ks, kb[c * n:(c + 1) * n])
+        np.testing.assert_array_equal(H.host(small_p), H.host(big_p)[c * n:(c + 1) * n])
+
+
 def test_watson_refnum_gridsearch_matches_reference_plugin(torch_cuda):
     """NODDI_Watson_refnum (the reference's numerical forms) on the GPU picks
     the same GridSearch combination as the oracle engine over the
