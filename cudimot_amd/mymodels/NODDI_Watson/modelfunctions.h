@@ -34,7 +34,13 @@ MACRO void watson_parts(T* P, T* CFP, const CDM_F64* vc, const CDM_F64* mc, T& e
   tsincos(P[3], &st, &ct);
   tsincos(P[4], &sp, &cp);
   T gn = CFP[0] * st * cp + CFP[1] * st * sp + CFP[2] * ct;
+#ifdef CDM_WATSON_REFERENCE_FORMS
   extra = exp_gpu(-CFP[3] * ((T)(vc[0] - vc[1]) * gn * gn + (T)vc[1]));
+#else
+  // the hardware 2^x form in single precision: its error is below the float
+  // rounding the argument already carries (exact exp in double)
+  extra = texp_fast(-CFP[3] * ((T)(vc[0] - vc[1]) * gn * gn + (T)vc[1]));
+#endif
   if (gn > (T)1) gn = (T)1;
   if (gn < (T)-1) gn = (T)-1;
   T sh[7];
